@@ -1,0 +1,721 @@
+// C-ABI of libdvr_hip.so: validation, parameter packing and kernel dispatch.
+//
+// Validation follows renderer::Renderer::checkInputs / checkOutput /
+// checkAdjointOutput (renderer/renderer.cpp:48-391) and checkBatch
+// (renderer/pytorch_utils.h:12-20), with the same messages where the
+// reference has them.  Unlike the reference (renderer/renderer_cuda.cpp:109-110,
+// renderer/renderer.cpp:407-409) launch failures are reported, never ignored.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dvr.h"
+#include "dvr_launch.hpp"
+#include "dvr_params.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Fail {
+    dvr_status code;
+};
+
+[[noreturn]] void fail(dvr_status code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    throw Fail{code};
+}
+
+void check_hip(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(DVR_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+const char* dtype_name(int d) { return d == DVR_FLOAT32 ? "torch.float32" : (d == DVR_INT32 ? "torch.int32" : "?"); }
+
+bool given(const dvr_tensor& t) { return t.data != nullptr || t.ndim > 0; }
+
+// CHECK_DIM, CHECK_SIZE, CHECK_DTYPE, CHECK_CUDA of renderer/pytorch_utils.h:5-11
+void check_dim(const dvr_tensor& t, const char* name, int d)
+{
+    if (!given(t)) fail(DVR_ERR_INVALID_ARGUMENT, "%s is not defined!", name);
+    if (t.ndim != d) fail(DVR_ERR_INVALID_ARGUMENT, "%s must be a tensor with %d dimensions, but has %d", name, d, t.ndim);
+}
+void check_size(const dvr_tensor& t, const char* name, int d, int64_t s)
+{
+    if (t.sizes[d] != s)
+        fail(DVR_ERR_INVALID_ARGUMENT, "%s must have %lld entries at dimension %d, but has %lld entries", name,
+             (long long)s, d, (long long)t.sizes[d]);
+}
+void check_dtype(const dvr_tensor& t, const char* name, int dt)
+{
+    if (t.dtype != dt)
+        fail(DVR_ERR_INVALID_ARGUMENT, "%s must be of type %s, but is %s", name, dtype_name(dt), dtype_name(t.dtype));
+}
+void check_device(const dvr_tensor& t, const char* name, int device)
+{
+    if (t.device != device)
+        fail(DVR_ERR_INVALID_ARGUMENT, "All tensors must be on the same device, but %s is on device %d (expected %d)",
+             name, t.device, device);
+}
+// checkBatch: size 1 broadcasts
+int check_batch(const dvr_tensor& t, const char* name, int B1, int idx = 0)
+{
+    const int B2 = (int)t.sizes[idx];
+    if (B2 == 1) return B1;
+    if (B1 == 1) return B2;
+    if (B1 != B2)
+        fail(DVR_ERR_INVALID_ARGUMENT,
+             "%s must be have the same batch size as other tensors, if batch>1. actual=%d, expected=%d", name, B2, B1);
+    return B1;
+}
+
+// broadcasting stride: a size-1 dimension gets stride 0 (renderer_tensor.cuh:140-144)
+int bstride(const dvr_tensor& t, int d)
+{
+    if (t.sizes[d] == 1) return 0;
+    return (int)t.strides[d];
+}
+
+// every element offset must fit into int32
+void check_offsets(const dvr_tensor& t, const char* name)
+{
+    int64_t maxo = 0;
+    for (int d = 0; d < t.ndim; ++d) {
+        if (t.sizes[d] <= 0) continue;
+        if (t.strides[d] < 0) fail(DVR_ERR_INVALID_ARGUMENT, "%s has a negative stride (unsupported)", name);
+        maxo += (t.sizes[d] - 1) * t.strides[d];
+    }
+    if (maxo >= (int64_t(1) << 31)) fail(DVR_ERR_INVALID_ARGUMENT, "%s is too large for 32-bit element offsets", name);
+}
+
+struct Checked {
+    dvr::RenderParams P;
+    int B, W, H, X, Y, Z;
+    int device;
+    int filter, camera, tfm, blend;
+};
+
+Checked check_inputs(const dvr_inputs* in)
+{
+    if (!in) fail(DVR_ERR_INVALID_ARGUMENT, "inputs is NULL");
+    Checked c{};
+    dvr::RenderParams& P = c.P;
+    std::memset(&P, 0, sizeof(P));
+    c.W = in->screen_width;
+    c.H = in->screen_height;
+    if (c.W < 0 || c.H < 0) fail(DVR_ERR_INVALID_ARGUMENT, "screen size must be non-negative");
+
+    const dvr_tensor& vol = in->volume;
+    check_dim(vol, "volume", 4);
+    c.device = vol.device;
+    if (c.device < 0)
+        fail(DVR_ERR_UNSUPPORTED,
+             "volume is a host (CPU) tensor: this library renders on the GPU only; move the inputs to the HIP device");
+    check_dtype(vol, "volume", DVR_FLOAT32);
+    if (in->volume_filter_mode != DVR_FILTER_NEAREST && in->volume_filter_mode != DVR_FILTER_TRILINEAR)
+        fail(DVR_ERR_UNSUPPORTED, "volume_filter_mode %d is not supported (Nearest=0, Trilinear=1)",
+             in->volume_filter_mode);
+    c.B = (int)vol.sizes[0];
+    c.X = (int)vol.sizes[1];
+    c.Y = (int)vol.sizes[2];
+    c.Z = (int)vol.sizes[3];
+    check_offsets(vol, "volume");
+    P.vol = (const float*)vol.data;
+    P.X = c.X; P.Y = c.Y; P.Z = c.Z;
+    P.vsb = bstride(vol, 0); P.vsx = bstride(vol, 1); P.vsy = bstride(vol, 2); P.vsz = bstride(vol, 3);
+
+    auto box = [&](const dvr_tensor& t, const float* value, const char* name, const float*& ptr, int& sb, int& sc,
+                   float* u) {
+        if (given(t)) {
+            check_dim(t, name, 2);
+            check_device(t, name, c.device);
+            check_size(t, name, 1, 3);
+            c.B = check_batch(t, name, c.B);
+            check_dtype(t, name, DVR_FLOAT32);
+            check_offsets(t, name);
+            ptr = (const float*)t.data;
+            sb = bstride(t, 0);
+            sc = (int)t.strides[1];
+        } else {
+            ptr = nullptr;
+            u[0] = value[0]; u[1] = value[1]; u[2] = value[2];
+        }
+    };
+    box(in->box_min, in->box_min_value, "boxMin", P.bmin, P.bminSb, P.bminSc, P.bminU);
+    box(in->box_size, in->box_size_value, "boxSize", P.bsize, P.bsizeSb, P.bsizeSc, P.bsizeU);
+
+    c.camera = in->camera_mode;
+    P.camMode = c.camera;
+    switch (c.camera) {
+    case DVR_CAMERA_RAY_START_DIR: {
+        const dvr_tensor& s = in->camera_ray_start;
+        const dvr_tensor& d = in->camera_ray_dir;
+        check_dim(s, "cameraRayStart", 4); check_device(s, "cameraRayStart", c.device);
+        check_size(s, "cameraRayStart", 1, c.H); check_size(s, "cameraRayStart", 2, c.W);
+        check_size(s, "cameraRayStart", 3, 3);
+        c.B = check_batch(s, "cameraRayStart", c.B); check_dtype(s, "cameraRayStart", DVR_FLOAT32);
+        check_dim(d, "cameraRayDir", 4); check_device(d, "cameraRayDir", c.device);
+        check_size(d, "cameraRayDir", 1, c.H); check_size(d, "cameraRayDir", 2, c.W);
+        check_size(d, "cameraRayDir", 3, 3);
+        c.B = check_batch(d, "cameraRayDir", c.B); check_dtype(d, "cameraRayDir", DVR_FLOAT32);
+        check_offsets(s, "cameraRayStart"); check_offsets(d, "cameraRayDir");
+        P.rs = (const float*)s.data;
+        P.rsSb = bstride(s, 0); P.rsSy = bstride(s, 1); P.rsSx = bstride(s, 2); P.rsSc = (int)s.strides[3];
+        P.rd = (const float*)d.data;
+        P.rdSb = bstride(d, 0); P.rdSy = bstride(d, 1); P.rdSx = bstride(d, 2); P.rdSc = (int)d.strides[3];
+    } break;
+    case DVR_CAMERA_INVERSE_VIEW_MATRIX: {
+        const dvr_tensor& m = in->camera_matrix;
+        if (given(m)) {
+            check_dim(m, "cameraMatrix", 3); check_device(m, "cameraMatrix", c.device);
+            check_size(m, "cameraMatrix", 1, 4); check_size(m, "cameraMatrix", 2, 4);
+            c.B = check_batch(m, "cameraMatrix", c.B); check_dtype(m, "cameraMatrix", DVR_FLOAT32);
+            check_offsets(m, "cameraMatrix");
+            P.cm = (const float*)m.data;
+            P.cmSb = bstride(m, 0); P.cmSr = (int)m.strides[1]; P.cmSc = (int)m.strides[2];
+        } else {
+            P.cm = nullptr;
+            for (int k = 0; k < 16; ++k) P.cmU[k] = in->camera_matrix_value[k];
+        }
+    } break;
+    case DVR_CAMERA_REFERENCE_FRAME: {
+        const dvr_tensor& m = in->camera_matrix;
+        check_dim(m, "cameraMatrix", 3); check_device(m, "cameraMatrix", c.device);
+        check_size(m, "cameraMatrix", 1, 3); check_size(m, "cameraMatrix", 2, 3);
+        c.B = check_batch(m, "cameraMatrix", c.B); check_dtype(m, "cameraMatrix", DVR_FLOAT32);
+        check_offsets(m, "cameraMatrix");
+        P.cm = (const float*)m.data;
+        P.cmSb = bstride(m, 0); P.cmSr = (int)m.strides[1]; P.cmSc = (int)m.strides[2];
+        P.fovY = in->camera_fov_y_radians;
+    } break;
+    default:
+        fail(DVR_ERR_INVALID_ARGUMENT, "unknown camera enum value");
+    }
+
+    const dvr_tensor& st = in->step_size;
+    if (given(st)) {
+        check_dim(st, "stepSize", 3); check_device(st, "stepSize", c.device);
+        c.B = check_batch(st, "stepSize", c.B);
+        check_batch(st, "stepSize-Height", c.H, 1);
+        check_batch(st, "stepSize-Width", c.W, 2);
+        check_dtype(st, "stepSize", DVR_FLOAT32);
+        check_offsets(st, "stepSize");
+        P.st = (const float*)st.data;
+        P.stSb = bstride(st, 0); P.stSy = bstride(st, 1); P.stSx = bstride(st, 2);
+    } else {
+        P.st = nullptr;
+        P.stU = in->step_size_value;
+        if (!(P.stU > 0.0f) || !std::isfinite(P.stU)) fail(DVR_ERR_INVALID_ARGUMENT, "step size must be positive and finite");
+    }
+
+    const dvr_tensor& tf = in->tf;
+    check_dim(tf, "tf", 3); check_device(tf, "tf", c.device);
+    c.B = check_batch(tf, "tf", c.B);
+    check_dtype(tf, "tf", DVR_FLOAT32);
+    c.tfm = in->tf_mode;
+    switch (c.tfm) {
+    case DVR_TF_IDENTITY: check_size(tf, "tf", 1, 1); check_size(tf, "tf", 2, 2); break;
+    case DVR_TF_TEXTURE: check_size(tf, "tf", 2, 4); break;
+    case DVR_TF_LINEAR:
+        check_size(tf, "tf", 2, 5);
+        if (tf.sizes[1] <= 1) fail(DVR_ERR_INVALID_ARGUMENT, "tensor 'tf' must have at least two control points in TFLinear-mode");
+        break;
+    case DVR_TF_GAUSSIAN: check_size(tf, "tf", 2, 6); break;
+    default: fail(DVR_ERR_UNSUPPORTED, "tf mode %d is not supported (Identity=0, Texture=1, Linear=2, Gaussian=3)", c.tfm);
+    }
+    if (tf.sizes[1] < 1) fail(DVR_ERR_INVALID_ARGUMENT, "tensor 'tf' must have at least one row");
+    check_offsets(tf, "tf");
+    P.tf = (const float*)tf.data;
+    P.R = (int)tf.sizes[1]; P.C = (int)tf.sizes[2];
+    P.tfSb = bstride(tf, 0); P.tfSr = (int)tf.strides[1]; P.tfSc = (int)tf.strides[2];
+
+    c.filter = in->volume_filter_mode;
+    c.blend = in->blend_mode;
+    if (c.blend != DVR_BLEND_BEER_LAMBERT && c.blend != DVR_BLEND_ALPHA)
+        fail(DVR_ERR_INVALID_ARGUMENT, "unknown blend enum value");
+    P.earlyOut = in->blending_early_out;
+    if (c.X < 2 || c.Y < 2 || c.Z < 2)
+        fail(DVR_ERR_INVALID_ARGUMENT, "volume must have at least 2 voxels per dimension, got %dx%dx%d", c.X, c.Y, c.Z);
+    P.W = c.W; P.H = c.H; P.B = c.B;
+    if ((int64_t)c.W * c.H * c.B >= (int64_t(1) << 31)) fail(DVR_ERR_INVALID_ARGUMENT, "B*H*W too large");
+    return c;
+}
+
+void check_outputs(const dvr_outputs* out, Checked& c)
+{
+    if (!out) fail(DVR_ERR_INVALID_ARGUMENT, "outputs is NULL");
+    const dvr_tensor& col = out->color;
+    check_dim(col, "color", 4); check_device(col, "color", c.device);
+    check_size(col, "color", 0, c.B); check_size(col, "color", 1, c.H); check_size(col, "color", 2, c.W);
+    check_size(col, "color", 3, 4); check_dtype(col, "color", DVR_FLOAT32);
+    check_offsets(col, "color");
+    const dvr_tensor& ti = out->termination_index;
+    check_dim(ti, "terminationIndex", 3); check_device(ti, "terminationIndex", c.device);
+    check_size(ti, "terminationIndex", 0, c.B); check_size(ti, "terminationIndex", 1, c.H);
+    check_size(ti, "terminationIndex", 2, c.W); check_dtype(ti, "terminationIndex", DVR_INT32);
+    check_offsets(ti, "terminationIndex");
+    c.P.color = (float*)col.data;
+    c.P.colSb = (int)col.strides[0]; c.P.colSy = (int)col.strides[1]; c.P.colSx = (int)col.strides[2];
+    c.P.colSc = (int)col.strides[3];
+    c.P.term = (int*)ti.data;
+    c.P.termSb = (int)ti.strides[0]; c.P.termSy = (int)ti.strides[1]; c.P.termSx = (int)ti.strides[2];
+}
+
+dvr::AdjointParams check_adjoint(const Checked& c, const dvr_outputs* fwd, const dvr_tensor* adj_color,
+                                 const dvr_adjoint_outputs* ao)
+{
+    dvr::AdjointParams A;
+    std::memset(&A, 0, sizeof(A));
+    if (!fwd) fail(DVR_ERR_INVALID_ARGUMENT, "outputs_from_forward is NULL");
+    const dvr_tensor& col = fwd->color;
+    check_dim(col, "output_color", 4); check_device(col, "output_color", c.device);
+    check_size(col, "output_color", 0, c.B); check_size(col, "output_color", 1, c.H);
+    check_size(col, "output_color", 2, c.W); check_size(col, "output_color", 3, 4);
+    check_dtype(col, "output_color", DVR_FLOAT32); check_offsets(col, "output_color");
+    const dvr_tensor& ti = fwd->termination_index;
+    check_dim(ti, "terminationIndex", 3); check_device(ti, "terminationIndex", c.device);
+    check_size(ti, "terminationIndex", 0, c.B); check_size(ti, "terminationIndex", 1, c.H);
+    check_size(ti, "terminationIndex", 2, c.W); check_dtype(ti, "terminationIndex", DVR_INT32);
+    check_offsets(ti, "terminationIndex");
+    A.fcol = (const float*)col.data;
+    A.fcSb = (int)col.strides[0]; A.fcSy = (int)col.strides[1]; A.fcSx = (int)col.strides[2]; A.fcSc = (int)col.strides[3];
+    A.fterm = (const int*)ti.data;
+    A.ftSb = (int)ti.strides[0]; A.ftSy = (int)ti.strides[1]; A.ftSx = (int)ti.strides[2];
+
+    if (!adj_color) fail(DVR_ERR_INVALID_ARGUMENT, "adj_color is NULL");
+    const dvr_tensor& ac = *adj_color;
+    check_dim(ac, "adj_color", 4); check_device(ac, "adj_color", c.device);
+    check_size(ac, "adj_color", 0, c.B); check_size(ac, "adj_color", 1, c.H);
+    check_size(ac, "adj_color", 2, c.W); check_size(ac, "adj_color", 3, 4);
+    check_dtype(ac, "adj_color", DVR_FLOAT32); check_offsets(ac, "adj_color");
+    A.acol = (const float*)ac.data;
+    A.acSb = (int)ac.strides[0]; A.acSy = (int)ac.strides[1]; A.acSx = (int)ac.strides[2]; A.acSc = (int)ac.strides[3];
+
+    if (!ao) fail(DVR_ERR_INVALID_ARGUMENT, "adj_outputs is NULL");
+    if (ao->has_volume_derivatives) {
+        const dvr_tensor& t = ao->adj_volume;
+        check_dim(t, "adj_volume", 4); check_device(t, "adj_volume", c.device);
+        check_batch(t, "adj_volume", c.B);
+        check_size(t, "adj_volume", 1, c.X); check_size(t, "adj_volume", 2, c.Y); check_size(t, "adj_volume", 3, c.Z);
+        check_dtype(t, "adj_volume", DVR_FLOAT32); check_offsets(t, "adj_volume");
+        A.avol = (float*)t.data;
+        A.avSb = bstride(t, 0); A.avSx = bstride(t, 1); A.avSy = bstride(t, 2); A.avSz = bstride(t, 3);
+    }
+    if (ao->has_stepsize_derivatives) {
+        const dvr_tensor& t = ao->adj_stepsize;
+        check_dim(t, "adj_stepSize", 3); check_device(t, "adj_stepSize", c.device);
+        check_batch(t, "adj_stepSize", c.B);
+        check_batch(t, "adj_stepSize", c.H, 1);
+        check_batch(t, "adj_stepSize", c.W, 2);
+        check_dtype(t, "adj_stepSize", DVR_FLOAT32); check_offsets(t, "adj_stepSize");
+        A.ast = (float*)t.data;
+        A.astSb = bstride(t, 0); A.astSy = bstride(t, 1); A.astSx = bstride(t, 2);
+        A.stepBroadcast = (c.B != t.sizes[0] || c.H != t.sizes[1] || c.W != t.sizes[2]) ? 1 : 0;
+    }
+    if (ao->has_camera_derivatives) {
+        const dvr_tensor& s = ao->adj_camera_ray_start;
+        const dvr_tensor& d = ao->adj_camera_ray_dir;
+        check_dim(s, "adj_cameraRayStart", 4); check_device(s, "adj_cameraRayStart", c.device);
+        check_batch(s, "adj_cameraRayStart", c.B);
+        check_size(s, "adj_cameraRayStart", 1, c.H); check_size(s, "adj_cameraRayStart", 2, c.W);
+        check_size(s, "adj_cameraRayStart", 3, 3); check_dtype(s, "adj_cameraRayStart", DVR_FLOAT32);
+        check_dim(d, "adj_cameraRayDir", 4); check_device(d, "adj_cameraRayDir", c.device);
+        check_batch(d, "adj_cameraRayDir", c.B);
+        check_size(d, "adj_cameraRayDir", 1, c.H); check_size(d, "adj_cameraRayDir", 2, c.W);
+        check_size(d, "adj_cameraRayDir", 3, 3); check_dtype(d, "adj_cameraRayDir", DVR_FLOAT32);
+        check_offsets(s, "adj_cameraRayStart"); check_offsets(d, "adj_cameraRayDir");
+        A.ars = (float*)s.data;
+        A.arsSb = bstride(s, 0); A.arsSy = bstride(s, 1); A.arsSx = bstride(s, 2); A.arsSc = (int)s.strides[3];
+        A.ard = (float*)d.data;
+        A.ardSb = bstride(d, 0); A.ardSy = bstride(d, 1); A.ardSx = bstride(d, 2); A.ardSc = (int)d.strides[3];
+        A.camBroadcast = (c.B != s.sizes[0] || c.B != d.sizes[0]) ? 1 : 0;
+    }
+    if (ao->has_tf_derivatives) {
+        const dvr_tensor& t = ao->adj_tf;
+        check_dim(t, "adj_tf", 3); check_device(t, "adj_tf", c.device);
+        check_batch(t, "adj_tf", c.B);
+        check_size(t, "adj_tf", 1, c.P.R); check_size(t, "adj_tf", 2, c.P.C);
+        check_dtype(t, "adj_tf", DVR_FLOAT32); check_offsets(t, "adj_tf");
+        A.atf = (float*)t.data;
+        A.atfSb = bstride(t, 0); A.atfSr = (int)t.strides[1]; A.atfSc = (int)t.strides[2];
+        A.tfDelayed = ao->tf_delayed_accumulation ? 1 : 0;
+    }
+    if (!ao->has_tf_derivatives && !ao->has_camera_derivatives && !ao->has_stepsize_derivatives &&
+        !ao->has_volume_derivatives)
+        fail(DVR_ERR_NO_GRADIENT, "At least one adjoint variable must be activated!");
+    return A;
+}
+
+// ---------------------------------------------------------------------------
+// workspace: grow-only device buffer per device for checkpoint scratch
+// ---------------------------------------------------------------------------
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::vector<Workspace> g_ws;
+
+void* workspace(int device, size_t bytes, hipStream_t stream)
+{
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if ((int)g_ws.size() <= device) g_ws.resize(device + 1);
+    Workspace& w = g_ws[device];
+    if (w.bytes >= bytes) return w.ptr;
+    if (w.ptr) {
+        // earlier work on this stream may still read the old buffer
+        check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check_hip(hipFree(w.ptr), "hipFree");
+        w.ptr = nullptr;
+        w.bytes = 0;
+    }
+    const size_t want = std::max(bytes, size_t(1) << 20);
+    check_hip(hipMalloc(&w.ptr, want), "hipMalloc(workspace)");
+    w.bytes = want;
+    return w.ptr;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        check_hip(hipGetDevice(&prev), "hipGetDevice");
+        if (prev != dev) check_hip(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+bool tiled_ok(const Checked& c) { return (int64_t)c.P.R * c.P.C <= dvr::kMaxTfLds; }
+
+// checkpoints per ray: enough for the longest ray when box and step are
+// uniform values, else 1024 samples' worth (longer rays recompute from the
+// last checkpoint).
+int checkpoint_cap(const Checked& c, int K)
+{
+    int cap;
+    if (!c.P.st && !c.P.bsize) {
+        const double d = std::sqrt(double(c.P.bsizeU[0]) * c.P.bsizeU[0] + double(c.P.bsizeU[1]) * c.P.bsizeU[1] +
+                                   double(c.P.bsizeU[2]) * c.P.bsizeU[2]);
+        const double n = d / double(c.P.stU) + 2.0;
+        cap = (int)std::ceil(n / K);
+    } else {
+        cap = 1024 / K;
+    }
+    return std::max(1, std::min(cap, 1 << 14));
+}
+
+int default_k(const Checked&) { return 16; }
+
+template <class F> dvr_status guarded(F&& f)
+{
+    try {
+        g_error.clear();
+        f();
+        return DVR_OK;
+    } catch (const Fail& e) {
+        return e.code;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return DVR_ERR_HIP;
+    }
+}
+
+void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forward_options* opt, void* stream)
+{
+    Checked c = check_inputs(inputs);
+    check_outputs(outputs, c);
+    DeviceGuard g(c.device);
+    hipStream_t s = (hipStream_t)stream;
+    int path = opt ? opt->path : DVR_PATH_AUTO;
+    const int K = opt ? opt->checkpoint_interval : 0;
+    if (path == DVR_PATH_AUTO) path = tiled_ok(c) ? DVR_PATH_TILED : DVR_PATH_PLAIN;
+    if (path == DVR_PATH_TILED && !tiled_ok(c))
+        fail(DVR_ERR_UNSUPPORTED, "tiled path needs R*C <= %d transfer-function entries", dvr::kMaxTfLds);
+    if (K > 0) {
+        if (path != DVR_PATH_TILED) fail(DVR_ERR_UNSUPPORTED, "checkpoints are written by the tiled forward only");
+        if (K != 8 && K != 16 && K != 32) fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 8, 16 or 32");
+        const dvr_tensor& ck = opt->checkpoints;
+        const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * checkpoint_cap(c, K) * dvr::kBlock;
+        if (!given(ck) || ck.dtype != DVR_FLOAT32 || ck.device != c.device)
+            fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints must be a float32 tensor on the render device");
+        int64_t numel = 1;
+        for (int d = 0; d < ck.ndim; ++d) numel *= ck.sizes[d];
+        if (numel < need) fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints needs %lld elements, has %lld", (long long)need, (long long)numel);
+        if (ck.ndim > 0 && ck.strides[ck.ndim - 1] != 1 && numel > 1)
+            fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints must be contiguous");
+    }
+    if (path == DVR_PATH_PLAIN) {
+        check_hip(dvr::launch_forward_plain(c.P, c.filter, c.tfm, c.blend, s), "forward (plain) launch");
+    } else {
+        dvr::CkptParams CK{nullptr, K > 0 ? K : 16, 0};
+        if (K > 0) {
+            CK.buf = (float*)opt->checkpoints.data;
+            CK.cap = checkpoint_cap(c, K);
+        }
+        check_hip(dvr::launch_forward_tiled(c.P, CK, c.filter, c.tfm, c.blend, s), "forward (tiled) launch");
+    }
+}
+
+void do_adjoint(const dvr_inputs* inputs, const dvr_outputs* fwd, const dvr_tensor* adj_color,
+                dvr_adjoint_outputs* ao, const dvr_adjoint_options* opt, void* stream)
+{
+    Checked c = check_inputs(inputs);
+    dvr::AdjointParams A = check_adjoint(c, fwd, adj_color, ao);
+    DeviceGuard g(c.device);
+    hipStream_t s = (hipStream_t)stream;
+    int method = opt ? opt->method : DVR_ADJOINT_AUTO;
+    int path = opt ? opt->path : DVR_PATH_AUTO;
+    if (method == DVR_ADJOINT_AUTO) method = tiled_ok(c) ? DVR_ADJOINT_CHECKPOINT : DVR_ADJOINT_INVERSION;
+    if (method == DVR_ADJOINT_INVERSION) {
+        if (path == DVR_PATH_TILED) fail(DVR_ERR_UNSUPPORTED, "the inversion adjoint runs on the plain path only");
+        check_hip(dvr::launch_adjoint_plain(c.P, A, c.filter, c.tfm, c.blend, s), "adjoint (plain) launch");
+        return;
+    }
+    if (method != DVR_ADJOINT_CHECKPOINT) fail(DVR_ERR_INVALID_ARGUMENT, "unknown adjoint method %d", method);
+    if (path == DVR_PATH_PLAIN) fail(DVR_ERR_UNSUPPORTED, "the checkpointed adjoint runs on the tiled path only");
+    if (!tiled_ok(c)) fail(DVR_ERR_UNSUPPORTED, "checkpointed adjoint needs R*C <= %d", dvr::kMaxTfLds);
+    int K = opt ? opt->checkpoint_interval : 0;
+    if (K == 0) K = default_k(c);
+    if (K != 8 && K != 16 && K != 32) fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 8, 16 or 32");
+    dvr::CkptParams CK{nullptr, K, checkpoint_cap(c, K)};
+    const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * CK.cap * dvr::kBlock;
+    bool fromFwd = false;
+    if (opt && given(opt->checkpoints)) {
+        const dvr_tensor& ck = opt->checkpoints;
+        int64_t numel = 1;
+        for (int d = 0; d < ck.ndim; ++d) numel *= ck.sizes[d];
+        if (ck.dtype != DVR_FLOAT32 || ck.device != c.device || numel < need)
+            fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints do not match these inputs and checkpoint_interval");
+        CK.buf = (float*)ck.data;
+        fromFwd = true;
+    } else {
+        CK.buf = (float*)workspace(c.device, size_t(need) * sizeof(float), s);
+    }
+    dvr::LossParams L;
+    std::memset(&L, 0, sizeof(L));
+    check_hip(dvr::launch_adjoint_tiled(c.P, A, CK, L, false, fromFwd, c.filter, c.tfm, c.blend, s),
+              "adjoint (tiled) launch");
+}
+
+void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss, dvr_adjoint_outputs* ao,
+              int K, void* stream)
+{
+    Checked c = check_inputs(inputs);
+    check_outputs(outputs, c);
+    if (!loss) fail(DVR_ERR_INVALID_ARGUMENT, "loss is NULL");
+    if (loss->kind != DVR_LOSS_L1 && loss->kind != DVR_LOSS_L2)
+        fail(DVR_ERR_INVALID_ARGUMENT, "loss kind must be DVR_LOSS_L1 or DVR_LOSS_L2");
+    const dvr_tensor& tg = loss->target;
+    check_dim(tg, "target", 4); check_device(tg, "target", c.device);
+    check_size(tg, "target", 0, c.B); check_size(tg, "target", 1, c.H); check_size(tg, "target", 2, c.W);
+    check_size(tg, "target", 3, 4); check_dtype(tg, "target", DVR_FLOAT32); check_offsets(tg, "target");
+    // the adjoint reads colour/termination/adj_color from nowhere in fused mode;
+    // validate its outputs against the forward outputs just checked
+    dvr_tensor dummy = outputs->color;
+    dvr::AdjointParams A = check_adjoint(c, outputs, &dummy, ao);
+    dvr::LossParams L;
+    std::memset(&L, 0, sizeof(L));
+    L.kind = loss->kind;
+    L.target = (const float*)tg.data;
+    L.tgSb = (int)tg.strides[0]; L.tgSy = (int)tg.strides[1]; L.tgSx = (int)tg.strides[2]; L.tgSc = (int)tg.strides[3];
+    const double n = double(c.B) * c.H * c.W * 4.0;
+    L.scale = loss->scale != 0.0f ? loss->scale : (n > 0 ? float(1.0 / n) : 0.0f);
+    if (given(loss->loss)) {
+        const dvr_tensor& lo = loss->loss;
+        check_device(lo, "loss", c.device); check_dtype(lo, "loss", DVR_FLOAT32);
+        L.lossOut = (float*)lo.data;
+    }
+    if (!tiled_ok(c)) fail(DVR_ERR_UNSUPPORTED, "fused rendering needs R*C <= %d", dvr::kMaxTfLds);
+    if (K == 0) K = default_k(c);
+    if (K != 8 && K != 16 && K != 32) fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 8, 16 or 32");
+    DeviceGuard g(c.device);
+    hipStream_t s = (hipStream_t)stream;
+    dvr::CkptParams CK{nullptr, K, checkpoint_cap(c, K)};
+    const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * CK.cap * dvr::kBlock;
+    CK.buf = (float*)workspace(c.device, size_t(need) * sizeof(float), s);
+    check_hip(dvr::launch_adjoint_tiled(c.P, A, CK, L, true, false, c.filter, c.tfm, c.blend, s), "fused launch");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// exported entry points
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* dvr_last_error(void) { return g_error.c_str(); }
+int32_t dvr_use_double_precision(void) { return 0; }
+const char* dvr_version(void) { return "dvr-mi355x 0.1 (gfx950)"; }
+
+dvr_status dvr_sync(void)
+{
+    return guarded([&] { check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
+}
+
+dvr_status dvr_render_forward(const dvr_inputs* inputs, dvr_outputs* outputs, void* stream)
+{
+    return guarded([&] { do_forward(inputs, outputs, nullptr, stream); });
+}
+
+dvr_status dvr_render_forward_ex(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forward_options* options,
+                                 void* stream)
+{
+    return guarded([&] { do_forward(inputs, outputs, options, stream); });
+}
+
+dvr_status dvr_render_adjoint(const dvr_inputs* inputs, const dvr_outputs* outputs_from_forward,
+                              const dvr_tensor* adj_color, dvr_adjoint_outputs* adj_outputs, void* stream)
+{
+    return guarded([&] { do_adjoint(inputs, outputs_from_forward, adj_color, adj_outputs, nullptr, stream); });
+}
+
+dvr_status dvr_render_adjoint_ex(const dvr_inputs* inputs, const dvr_outputs* outputs_from_forward,
+                                 const dvr_tensor* adj_color, dvr_adjoint_outputs* adj_outputs,
+                                 const dvr_adjoint_options* options, void* stream)
+{
+    return guarded([&] { do_adjoint(inputs, outputs_from_forward, adj_color, adj_outputs, options, stream); });
+}
+
+dvr_status dvr_render_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss,
+                            dvr_adjoint_outputs* adj_outputs, int32_t checkpoint_interval, void* stream)
+{
+    return guarded([&] { do_fused(inputs, outputs, loss, adj_outputs, checkpoint_interval, stream); });
+}
+
+int64_t dvr_checkpoint_buffer_elements(const dvr_inputs* inputs, int32_t checkpoint_interval)
+{
+    if (checkpoint_interval <= 0) return 0;
+    int64_t r = -1;
+    guarded([&] {
+        Checked c = check_inputs(inputs);
+        r = (int64_t)dvr::tiled_num_tiles(c.P) * checkpoint_cap(c, checkpoint_interval) * dvr::kBlock;
+    });
+    return r;
+}
+
+dvr_status dvr_generate_rays(const dvr_tensor* viewport, float fov_y_radians, int32_t width, int32_t height,
+                             dvr_tensor* ray_start, dvr_tensor* ray_dir, void* stream)
+{
+    return guarded([&] {
+        if (!viewport || !ray_start || !ray_dir) fail(DVR_ERR_INVALID_ARGUMENT, "NULL argument");
+        const dvr_tensor& v = *viewport;
+        check_dim(v, "viewport", 3); check_size(v, "viewport", 1, 3); check_size(v, "viewport", 2, 3);
+        check_dtype(v, "viewport", DVR_FLOAT32);
+        if (v.device < 0) fail(DVR_ERR_UNSUPPORTED, "viewport is a host tensor: this library runs on the GPU only");
+        const int B = (int)v.sizes[0];
+        for (dvr_tensor* t : {ray_start, ray_dir}) {
+            const char* n = t == ray_start ? "rayStart" : "rayDir";
+            check_dim(*t, n, 4); check_device(*t, n, v.device); check_size(*t, n, 0, B);
+            check_size(*t, n, 1, height); check_size(*t, n, 2, width); check_size(*t, n, 3, 3);
+            check_dtype(*t, n, DVR_FLOAT32); check_offsets(*t, n);
+        }
+        check_offsets(v, "viewport");
+        DeviceGuard g(v.device);
+        const int rsS[4] = {(int)ray_start->strides[0], (int)ray_start->strides[1], (int)ray_start->strides[2],
+                            (int)ray_start->strides[3]};
+        const int rdS[4] = {(int)ray_dir->strides[0], (int)ray_dir->strides[1], (int)ray_dir->strides[2],
+                            (int)ray_dir->strides[3]};
+        check_hip(dvr::launch_generate_rays((const float*)v.data, (int)v.strides[0], (int)v.strides[1],
+                                            (int)v.strides[2], B, height, width, fov_y_radians,
+                                            (float*)ray_start->data, rsS, (float*)ray_dir->data, rdS,
+                                            (hipStream_t)stream),
+                  "generate_rays launch");
+    });
+}
+
+dvr_status dvr_generate_rays_adjoint(const dvr_tensor* viewport, float fov_y_radians, int32_t width, int32_t height,
+                                     const dvr_tensor* adj_ray_start, const dvr_tensor* adj_ray_dir,
+                                     dvr_tensor* adj_viewport, void* stream)
+{
+    return guarded([&] {
+        if (!viewport || !adj_ray_start || !adj_ray_dir || !adj_viewport) fail(DVR_ERR_INVALID_ARGUMENT, "NULL argument");
+        const dvr_tensor& v = *viewport;
+        check_dim(v, "viewport", 3); check_size(v, "viewport", 1, 3); check_size(v, "viewport", 2, 3);
+        check_dtype(v, "viewport", DVR_FLOAT32);
+        if (v.device < 0) fail(DVR_ERR_UNSUPPORTED, "viewport is a host tensor: this library runs on the GPU only");
+        const int B = (int)v.sizes[0];
+        for (const dvr_tensor* t : {adj_ray_start, adj_ray_dir}) {
+            const char* n = t == adj_ray_start ? "grad_rayStart" : "grad_rayDir";
+            check_dim(*t, n, 4); check_device(*t, n, v.device); check_size(*t, n, 0, B);
+            check_size(*t, n, 1, height); check_size(*t, n, 2, width); check_size(*t, n, 3, 3);
+            check_dtype(*t, n, DVR_FLOAT32); check_offsets(*t, n);
+        }
+        const dvr_tensor& o = *adj_viewport;
+        check_dim(o, "adj_viewport", 3); check_device(o, "adj_viewport", v.device); check_size(o, "adj_viewport", 0, B);
+        check_size(o, "adj_viewport", 1, 3); check_size(o, "adj_viewport", 2, 3);
+        check_dtype(o, "adj_viewport", DVR_FLOAT32);
+        check_offsets(v, "viewport"); check_offsets(o, "adj_viewport");
+        DeviceGuard g(v.device);
+        const int aS[4] = {(int)adj_ray_start->strides[0], (int)adj_ray_start->strides[1],
+                           (int)adj_ray_start->strides[2], (int)adj_ray_start->strides[3]};
+        const int dS[4] = {(int)adj_ray_dir->strides[0], (int)adj_ray_dir->strides[1], (int)adj_ray_dir->strides[2],
+                           (int)adj_ray_dir->strides[3]};
+        check_hip(dvr::launch_generate_rays_adjoint((const float*)v.data, (int)v.strides[0], (int)v.strides[1],
+                                                    (int)v.strides[2], B, height, width, fov_y_radians,
+                                                    (const float*)adj_ray_start->data, aS,
+                                                    (const float*)adj_ray_dir->data, dS, (float*)o.data,
+                                                    (int)o.strides[0], (int)o.strides[1], (int)o.strides[2],
+                                                    (hipStream_t)stream),
+                  "generate_rays_adjoint launch");
+    });
+}
+
+struct dvr_timer {
+    hipEvent_t start, stop;
+};
+
+dvr_status dvr_timer_create(dvr_timer** timer)
+{
+    return guarded([&] {
+        if (!timer) fail(DVR_ERR_INVALID_ARGUMENT, "NULL timer");
+        dvr_timer* t = new dvr_timer{};
+        check_hip(hipEventCreate(&t->start), "hipEventCreate");
+        check_hip(hipEventCreate(&t->stop), "hipEventCreate");
+        *timer = t;
+    });
+}
+dvr_status dvr_timer_destroy(dvr_timer* t)
+{
+    return guarded([&] {
+        if (!t) return;
+        (void)hipEventDestroy(t->start);
+        (void)hipEventDestroy(t->stop);
+        delete t;
+    });
+}
+dvr_status dvr_timer_start(dvr_timer* t, void* stream)
+{
+    return guarded([&] { check_hip(hipEventRecord(t->start, (hipStream_t)stream), "hipEventRecord"); });
+}
+dvr_status dvr_timer_stop(dvr_timer* t, void* stream)
+{
+    return guarded([&] { check_hip(hipEventRecord(t->stop, (hipStream_t)stream), "hipEventRecord"); });
+}
+dvr_status dvr_timer_elapsed_ms(dvr_timer* t, float* ms)
+{
+    return guarded([&] {
+        check_hip(hipEventSynchronize(t->stop), "hipEventSynchronize");
+        check_hip(hipEventElapsedTime(ms, t->start, t->stop), "hipEventElapsedTime");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,473 @@
+// Device building blocks of the ray marcher for gfx950 (wave64).
+//
+// Semantics follow DiffDVR (paths relative to that repository):
+//   camera modes      renderer/renderer_camera.cuh:12-61
+//   slab test         renderer/renderer_utils.cuh:60-74 (tmin clamped to 0, renderer_kernels.cuh:48)
+//   interpolation     renderer/renderer_interpolation.cuh:33-389
+//   transfer function renderer/renderer_tf.cuh:59-648
+//   blending          renderer/renderer_blending.cuh:36-165
+// The implementation is this project's own: plain float math, accurate
+// expf/tanf (no fast-math intrinsics), 32-bit offsets.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "dvr_params.hpp"
+
+namespace dvr {
+
+struct f3 { float x, y, z; };
+struct f4 { float x, y, z, w; };
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f4 mk4(float x, float y, float z, float w) { return f4{x, y, z, w}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 mul3(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 div3(f3 a, f3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
+__device__ __forceinline__ f3 scl3(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b)
+{
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ f3 normalize3(f3 v) { return scl3(v, rsqrtf(dot3(v, v))); }
+__device__ __forceinline__ float lerpf_(float a, float b, float t) { return a + t * (b - a); }
+// ternary min/max as in the reference (helper_math.cuh:86-104)
+__device__ __forceinline__ float tmin_(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float tmax_(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float fclamp(float f, float a, float b) { return fmaxf(a, fminf(f, b)); }
+__device__ __forceinline__ int iclamp(int f, int a, int b) { return max(a, min(f, b)); }
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Workgroup index -> logical tile index so that each XCD (workgroups are
+// dispatched round-robin over the 8 XCDs) owns one contiguous range of tiles:
+// neighbouring tiles then share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
+{
+    const int xcd = bid % kNumXcd, slot = bid / kNumXcd;
+    const int base = nblocks / kNumXcd, rem = nblocks % kNumXcd;
+    return xcd * base + min(xcd, rem) + slot;
+}
+
+// ---------------------------------------------------------------------------
+// Per-ray setup (renderer_kernels.cuh:31-48)
+// ---------------------------------------------------------------------------
+struct Ray {
+    f3 start, dir;
+    f3 bmin, voxel, invVoxel;
+    float step, t0, t1;
+};
+
+__device__ __forceinline__ f3 matmul_proj(const float m[16], float x, float y, float z)
+{
+    const float w0 = m[0] * x + m[1] * y + m[2] * z + m[3];
+    const float w1 = m[4] * x + m[5] * y + m[6] * z + m[7];
+    const float w2 = m[8] * x + m[9] * y + m[10] * z + m[11];
+    const float w3 = m[12] * x + m[13] * y + m[14] * z + m[15];
+    return mk3(w0 / w3, w1 / w3, w2 / w3);
+}
+
+// Camera from the standard parameter block (renderer_camera.cuh:12-61).
+__device__ __forceinline__ void camera_ray(const RenderParams& P, int x, int y, int b, f3& start, f3& dir)
+{
+    if (P.camMode == 0) {  // RayStartDir
+        const float* s = P.rs + b * P.rsSb + y * P.rsSy + x * P.rsSx;
+        const float* d = P.rd + b * P.rdSb + y * P.rdSy + x * P.rdSx;
+        start = mk3(s[0], s[P.rsSc], s[2 * P.rsSc]);
+        dir = mk3(d[0], d[P.rdSc], d[2 * P.rdSc]);
+    } else if (P.camMode == 1) {  // InverseViewMatrix
+        float m[16];
+        if (P.cm) {
+            const float* c = P.cm + b * P.cmSb;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) m[4 * r + k] = c[r * P.cmSr + k * P.cmSc];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) m[k] = P.cmU[k];
+        }
+        const float fx = 2.0f * (x + 0.5f) / float(P.W) - 1.0f;
+        const float fy = 2.0f * (y + 0.5f) / float(P.H) - 1.0f;
+        start = matmul_proj(m, fx, fy, 0.0f);
+        const f3 end = matmul_proj(m, fx, fy, 0.9f);
+        dir = normalize3(sub3(end, start));
+    } else {  // ReferenceFrame
+        const float* c = P.cm + b * P.cmSb;
+        const f3 eye = mk3(c[0], c[P.cmSc], c[2 * P.cmSc]);
+        const f3 right = mk3(c[P.cmSr], c[P.cmSr + P.cmSc], c[P.cmSr + 2 * P.cmSc]);
+        const f3 up = mk3(c[2 * P.cmSr], c[2 * P.cmSr + P.cmSc], c[2 * P.cmSr + 2 * P.cmSc]);
+        const float fovX = P.fovY * float(P.W) / float(P.H);
+        const float tanX = tanf(fovX * 0.5f), tanY = tanf(P.fovY * 0.5f);
+        const f3 front = cross3(up, right);
+        const float fx = 2.0f * (x + 0.5f) / float(P.W) - 1.0f;
+        const float fy = 2.0f * (y + 0.5f) / float(P.H) - 1.0f;
+        start = eye;
+        dir = normalize3(add3(add3(front, scl3(right, fx * tanX)), scl3(up, fy * tanY)));
+    }
+}
+
+// Slab test with the box given as min/size (renderer_utils.cuh:60-74).
+__device__ __forceinline__ void slab(f3 start, f3 dir, f3 bmin, f3 bsize, float& t0, float& t1)
+{
+    const float ix = 1.0f / dir.x, iy = 1.0f / dir.y, iz = 1.0f / dir.z;
+    const float a1 = (bmin.x - start.x) * ix, a2 = (bmin.x + bsize.x - start.x) * ix;
+    const float a3 = (bmin.y - start.y) * iy, a4 = (bmin.y + bsize.y - start.y) * iy;
+    const float a5 = (bmin.z - start.z) * iz, a6 = (bmin.z + bsize.z - start.z) * iz;
+    t0 = tmax_(tmax_(tmin_(a1, a2), tmin_(a3, a4)), tmin_(a5, a6));
+    t1 = tmin_(tmin_(tmax_(a1, a2), tmax_(a3, a4)), tmax_(a5, a6));
+}
+
+__device__ __forceinline__ Ray setup_ray(const RenderParams& P, int x, int y, int b)
+{
+    Ray r;
+    f3 bsize;
+    if (P.bmin) {
+        const float* p = P.bmin + b * P.bminSb;
+        r.bmin = mk3(p[0], p[P.bminSc], p[2 * P.bminSc]);
+    } else {
+        r.bmin = mk3(P.bminU[0], P.bminU[1], P.bminU[2]);
+    }
+    if (P.bsize) {
+        const float* p = P.bsize + b * P.bsizeSb;
+        bsize = mk3(p[0], p[P.bsizeSc], p[2 * P.bsizeSc]);
+    } else {
+        bsize = mk3(P.bsizeU[0], P.bsizeU[1], P.bsizeU[2]);
+    }
+    r.voxel = div3(bsize, mk3(float(P.X - 1), float(P.Y - 1), float(P.Z - 1)));
+    r.invVoxel = mk3(1.0f / r.voxel.x, 1.0f / r.voxel.y, 1.0f / r.voxel.z);
+    camera_ray(P, x, y, b, r.start, r.dir);
+    r.step = P.st ? P.st[b * P.stSb + y * P.stSy + x * P.stSx] : P.stU;
+    slab(r.start, r.dir, r.bmin, bsize, r.t0, r.t1);
+    r.t0 = tmax_(r.t0, 0.0f);
+    // Degenerate rays (zero direction, NaN slab bounds) and non-positive or
+    // non-finite steps would never leave the march loop: they sample nothing.
+    if (!(r.t1 >= r.t0) || !isfinite(r.t0) || !isfinite(r.t1) || !(r.step > 0.0f) || !isfinite(r.step)) {
+        r.t0 = 0.0f;
+        r.t1 = -1.0f;
+        if (!(r.step > 0.0f) || !isfinite(r.step)) r.step = 1.0f;
+    }
+    return r;
+}
+
+// volume-space sample position of parameter t: ((start + t dir) - bmin) / voxel
+__device__ __forceinline__ f3 sample_pos(const Ray& r, float t)
+{
+    const f3 w = mk3(r.start.x + t * r.dir.x, r.start.y + t * r.dir.y, r.start.z + t * r.dir.z);
+    return mul3(sub3(w, r.bmin), r.invVoxel);
+}
+
+// ---------------------------------------------------------------------------
+// Volume access (one batch slice)
+// ---------------------------------------------------------------------------
+struct Vol {
+    const float* p;   // batch slice base
+    int X, Y, Z;
+    int sx, sy, sz;
+};
+
+__device__ __forceinline__ Vol make_vol(const RenderParams& P, int b)
+{
+    return Vol{P.vol + b * P.vsb, P.X, P.Y, P.Z, P.vsx, P.vsy, P.vsz};
+}
+
+// Trilinear cell cache: the 8 corners of the current cell are reloaded only
+// when the (truncated) cell index changes (renderer_interpolation.cuh:218-225).
+struct Cell {
+    int ix, iy, iz;
+    float d[8];  // bit order (x<<2)|(y<<1)|z
+};
+
+__device__ __forceinline__ void cell_invalidate(Cell& c) { c.ix = c.iy = c.iz = -0x40000000; }
+
+__device__ __forceinline__ void cell_load(const Vol& V, int ix, int iy, int iz, float d[8])
+{
+    const int lx = iclamp(ix, 0, V.X - 1), hx = iclamp(ix + 1, 0, V.X - 1);
+    const int ly = iclamp(iy, 0, V.Y - 1), hy = iclamp(iy + 1, 0, V.Y - 1);
+    const int lz = iclamp(iz, 0, V.Z - 1), hz = iclamp(iz + 1, 0, V.Z - 1);
+    const int ox0 = lx * V.sx, ox1 = hx * V.sx, oy0 = ly * V.sy, oy1 = hy * V.sy, oz0 = lz * V.sz, oz1 = hz * V.sz;
+    d[0] = V.p[ox0 + oy0 + oz0];
+    d[1] = V.p[ox0 + oy0 + oz1];
+    d[2] = V.p[ox0 + oy1 + oz0];
+    d[3] = V.p[ox0 + oy1 + oz1];
+    d[4] = V.p[ox1 + oy0 + oz0];
+    d[5] = V.p[ox1 + oy0 + oz1];
+    d[6] = V.p[ox1 + oy1 + oz0];
+    d[7] = V.p[ox1 + oy1 + oz1];
+}
+
+__device__ __forceinline__ float tri_lerp(const float d[8], float fx, float fy, float fz)
+{
+    return lerpf_(lerpf_(lerpf_(d[0], d[4], fx), lerpf_(d[2], d[6], fx), fy),
+                  lerpf_(lerpf_(d[1], d[5], fx), lerpf_(d[3], d[7], fx), fy), fz);
+}
+
+// Nearest neighbour index (renderer_interpolation.cuh:38-50)
+__device__ __forceinline__ int nearest_offset(const Vol& V, f3 p)
+{
+    const float eps = 1e-7f;
+    const float cx = fclamp(p.x, eps, float(V.X) - (1.0f + eps));
+    const float cy = fclamp(p.y, eps, float(V.Y) - (1.0f + eps));
+    const float cz = fclamp(p.z, eps, float(V.Z) - (1.0f + eps));
+    return int(roundf(cx)) * V.sx + int(roundf(cy)) * V.sy + int(roundf(cz)) * V.sz;
+}
+
+
+template <int FILTER>
+__device__ __forceinline__ float fetch(const Vol& V, f3 p, Cell& c)
+{
+    if constexpr (FILTER == 0) {
+        return V.p[nearest_offset(V, p)];
+    } else {
+        const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
+        if (ix != c.ix || iy != c.iy || iz != c.iz) {
+            cell_load(V, ix, iy, iz, c.d);
+            c.ix = ix; c.iy = iy; c.iz = iz;
+        }
+        return tri_lerp(c.d, p.x - float(ix), p.y - float(iy), p.z - float(iz));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transfer functions.  `tf` points at row 0 of this batch's R x C table with
+// row stride sr and column stride sc (global or LDS memory).
+// ---------------------------------------------------------------------------
+struct Tf {
+    const float* p;
+    int R, C, sr, sc;
+};
+
+template <int TFM>
+__device__ __forceinline__ f4 tf_eval(const Tf& T, float density)
+{
+    if constexpr (TFM == 0) {  // identity (renderer_tf.cuh:62-73)
+        const float d = fclamp(density, 0.0f, 1.0f);
+        const float sa = T.p[0], sc = T.p[T.sc];
+        return mk4(d * sc, d * sc, d * sc, d * sa);
+    } else if constexpr (TFM == 1) {  // texture (:160-171)
+        const float d = density * float(T.R) - 0.5f;
+        const int di = int(floorf(d));
+        const float df = d - float(di);
+        const float* a = T.p + iclamp(di, 0, T.R - 1) * T.sr;
+        const float* c = T.p + iclamp(di + 1, 0, T.R - 1) * T.sr;
+        return mk4(lerpf_(a[0], c[0], df), lerpf_(a[T.sc], c[T.sc], df), lerpf_(a[2 * T.sc], c[2 * T.sc], df),
+                   lerpf_(a[3 * T.sc], c[3 * T.sc], df));
+    } else if constexpr (TFM == 2) {  // piecewise linear (:294-313)
+        int i;
+        for (i = 0; i < T.R - 2; ++i)
+            if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
+        const float* a = T.p + i * T.sr;
+        const float* c = a + T.sr;
+        const float p0 = a[4 * T.sc], p1 = c[4 * T.sc];
+        const float d2 = fclamp(density, p0, p1);
+        const float f = (d2 - p0) / (p1 - p0);
+        return mk4(lerpf_(a[0], c[0], f), lerpf_(a[T.sc], c[T.sc], f), lerpf_(a[2 * T.sc], c[2 * T.sc], f),
+                   lerpf_(a[3 * T.sc], c[3 * T.sc], f));
+    } else {  // sum of Gaussians, column 5 = sigma (:491-519)
+        f4 o = mk4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < T.R; ++i) {
+            const float* r = T.p + i * T.sr;
+            const float mu = r[4 * T.sc], s = r[5 * T.sc];
+            const float n = expf(-(density - mu) * (density - mu) / (2.0f * s * s));
+            o.x += r[0] * n; o.y += r[T.sc] * n; o.z += r[2 * T.sc] * n; o.w += r[3 * T.sc] * n;
+        }
+        return o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blending (renderer_blending.cuh:54-61, 118-125)
+// ---------------------------------------------------------------------------
+template <int BLEND>
+__device__ __forceinline__ float blend_alpha(float absorption, float step)
+{
+    if constexpr (BLEND == 0) return 1.0f - expf(-absorption * step);
+    else return tmin_(1.0f, absorption * step);
+}
+
+template <int BLEND>
+__device__ __forceinline__ void blend(f4& acc, const f4& s, float step)
+{
+    const float a = blend_alpha<BLEND>(s.w, step);
+    const float tr = 1.0f - acc.w;
+    acc.x = acc.x + tr * s.x * a;
+    acc.y = acc.y + tr * s.y * a;
+    acc.z = acc.z + tr * s.z * a;
+    acc.w = acc.w + tr * a;
+}
+
+// Blend adjoint for one sample given the accumulated alpha before it (a_in).
+// adj_acc.xyz is constant along the ray; adj_acc.w is propagated in place.
+// (renderer_blending.cuh:63-98 / :128-164 with a_in supplied instead of
+//  reconstructed.)
+template <int BLEND>
+__device__ __forceinline__ void blend_adjoint(float a_in, const f4& s, float step, f4& adj_acc, f4& adj_s,
+                                              float& adj_step)
+{
+    float alpha, e = 0.0f;
+    if constexpr (BLEND == 0) {
+        e = expf(-s.w * step);
+        alpha = 1.0f - e;
+    } else {
+        alpha = tmin_(1.0f - 1e-5f, s.w * step);
+    }
+    const float tin = 1.0f - a_in;
+    float adj_alpha = adj_acc.w * tin;
+    float adj_ain = adj_acc.w * (1.0f - alpha);
+    adj_alpha += adj_acc.x * (s.x - s.x * a_in) + adj_acc.y * (s.y - s.y * a_in) + adj_acc.z * (s.z - s.z * a_in);
+    adj_ain += adj_acc.x * (-s.x * alpha) + adj_acc.y * (-s.y * alpha) + adj_acc.z * (-s.z * alpha);
+    const float k = alpha * tin;
+    adj_s.x = adj_acc.x * k;
+    adj_s.y = adj_acc.y * k;
+    adj_s.z = adj_acc.z * k;
+    if constexpr (BLEND == 0) {
+        adj_s.w = adj_alpha * step * e;
+        adj_step = adj_alpha * s.w * e;
+    } else {
+        const bool lin = s.w * step < 1.0f;
+        adj_step = lin ? adj_alpha * s.w : 0.0f;
+        adj_s.w = lin ? adj_alpha * step : 0.0f;
+    }
+    adj_acc.w = adj_ain;
+}
+
+// Accumulated alpha before the sample from the one after it (reference
+// inversion, renderer_blending.cuh:75-77 / :140-142).
+template <int BLEND>
+__device__ __forceinline__ f4 blend_invert(const f4& acc_out, const f4& s, float step)
+{
+    float alpha;
+    if constexpr (BLEND == 0) alpha = 1.0f - expf(-s.w * step);
+    else alpha = tmin_(1.0f - 1e-5f, s.w * step);
+    const float a_in = (alpha - acc_out.w) / (alpha - 1.0f);
+    const float tr = 1.0f - a_in;
+    return mk4(acc_out.x - tr * s.x * alpha, acc_out.y - tr * s.y * alpha, acc_out.z - tr * s.z * alpha, a_in);
+}
+
+// ---------------------------------------------------------------------------
+// TF adjoint.  The parameter-gradient sink is a policy object with
+//   add(row, col, value)                       -- one entry
+//   add_pair<POS>(r0, w0, r1, w1, adj, p0, p1) -- rows r0/r1 get adj*w0 / adj*w1
+//                                                 in columns 0..3 (+ p0/p1 in
+//                                                 column 4 when POS)
+// so the same code feeds global atomics, LDS atomics or private arrays.
+// Returns d(density).  (renderer_tf.cuh:75-104, 173-220, 315-400, 562-614)
+// ---------------------------------------------------------------------------
+template <int TFM, bool WANT_TF, class Sink>
+__device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4& adj, Sink& sink)
+{
+    if constexpr (TFM == 0) {
+        const float dc = fclamp(density, 0.0f, 1.0f);
+        const float sa = T.p[0], sc = T.p[T.sc];
+        const float adj_dc = (adj.x * sc + adj.y * sc + adj.z * sc) + adj.w * sa;
+        if constexpr (WANT_TF) {
+            sink.add(0, 0, adj.w * dc);
+            sink.add(0, 1, adj.x * dc + adj.y * dc + adj.z * dc);
+        }
+        return (density > 0.0f && density < 1.0f) ? adj_dc : 0.0f;
+    } else if constexpr (TFM == 1) {
+        const float d = density * float(T.R) - 0.5f;
+        const int di = int(floorf(d));
+        const float df = d - float(di);
+        const int i0 = iclamp(di, 0, T.R - 1), i1 = iclamp(di + 1, 0, T.R - 1);
+        const float* a = T.p + i0 * T.sr;
+        const float* c = T.p + i1 * T.sr;
+        const float adj_df = adj.x * (c[0] - a[0]) + adj.y * (c[T.sc] - a[T.sc]) + adj.z * (c[2 * T.sc] - a[2 * T.sc]) +
+                             adj.w * (c[3 * T.sc] - a[3 * T.sc]);
+        if constexpr (WANT_TF) sink.template add_pair<false>(i0, 1.0f - df, i1, df, adj, 0.0f, 0.0f);
+        return adj_df * float(T.R);
+    } else if constexpr (TFM == 2) {
+        int i;
+        for (i = 0; i < T.R - 2; ++i)
+            if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
+        const float* a = T.p + i * T.sr;
+        const float* c = a + T.sr;
+        const float p0 = a[4 * T.sc], p1 = c[4 * T.sc];
+        const float d2 = fclamp(density, p0, p1);
+        const float f = (d2 - p0) / (p1 - p0);
+        const float adj_f = adj.x * (c[0] - a[0]) + adj.y * (c[T.sc] - a[T.sc]) + adj.z * (c[2 * T.sc] - a[2 * T.sc]) +
+                            adj.w * (c[3 * T.sc] - a[3 * T.sc]);
+        const float den = (p1 - p0) * (p1 - p0);
+        if constexpr (WANT_TF) {
+            const float adj_p0 = adj_f * (d2 - p1) / den;
+            const float adj_p1 = adj_f * (p0 - d2) / den;
+            sink.template add_pair<true>(i, 1.0f - f, i + 1, f, adj, adj_p0, adj_p1);
+        }
+        return adj_f / (p1 - p0);  // passes through the clamp (:350-351)
+    } else {
+        float adj_d = 0.0f;
+        for (int i = 0; i < T.R; ++i) {
+            const float* r = T.p + i * T.sr;
+            const float c0 = r[0], c1 = r[T.sc], c2 = r[2 * T.sc], c3 = r[3 * T.sc];
+            const float mu = r[4 * T.sc], s = r[5 * T.sc];
+            const float s2 = s * s;
+            const float n = expf(-(density - mu) * (density - mu) / (2.0f * s * s));
+            const float f1 = (mu - density) / s2 * n;
+            const float g = adj.x * (c0 * f1) + adj.y * (c1 * f1) + adj.z * (c2 * f1) + adj.w * (c3 * f1);
+            adj_d += g;
+            if constexpr (WANT_TF) {
+                const float f2 = f1 * (mu - density) / s;
+                sink.add(i, 0, adj.x * n);
+                sink.add(i, 1, adj.y * n);
+                sink.add(i, 2, adj.z * n);
+                sink.add(i, 3, adj.w * n);
+                sink.add(i, 4, -g);
+                sink.add(i, 5, adj.x * (c0 * f2) + adj.y * (c1 * f2) + adj.z * (c2 * f2) + adj.w * (c3 * f2));
+            }
+        }
+        return adj_d;
+    }
+}
+
+struct SinkNone {
+    __device__ void add(int, int, float) {}
+    template <bool POS> __device__ void add_pair(int, float, int, float, const f4&, float, float) {}
+};
+
+// Nearest-neighbour voxel indices (renderer_interpolation.cuh:38-50)
+__device__ __forceinline__ void nearest_index(f3 p, int X, int Y, int Z, int& ix, int& iy, int& iz)
+{
+    const float eps = 1e-7f;
+    ix = int(roundf(fclamp(p.x, eps, float(X) - (1.0f + eps))));
+    iy = int(roundf(fclamp(p.y, eps, float(Y) - (1.0f + eps))));
+    iz = int(roundf(fclamp(p.z, eps, float(Z) - (1.0f + eps))));
+}
+
+// Trilinear adjoint of one sample: corner weights w[8] (times adj_d) and the
+// position adjoint adj_cw - adj_fw (renderer_interpolation.cuh:360-387).
+// `d` are the corner densities (needed only for the position adjoint).
+template <bool WANT_POS>
+__device__ __forceinline__ void tri_adjoint(const float d[8], float fx, float fy, float fz, float adj_d, float w[8],
+                                            f3& adj_p)
+{
+    const float cx = fx, cy = fy, cz = fz;
+    const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+    w[0] = adj_d * (gx * gy * gz);
+    w[1] = adj_d * (gx * gy * cz);
+    w[2] = adj_d * (gx * cy * gz);
+    w[3] = adj_d * (gx * cy * cz);
+    w[4] = adj_d * (cx * gy * gz);
+    w[5] = adj_d * (cx * gy * cz);
+    w[6] = adj_d * (cx * cy * gz);
+    w[7] = adj_d * (cx * cy * cz);
+    if constexpr (WANT_POS) {
+        // d/dx of the trilinear form = sum over corners of +-(y-weight)(z-weight) d_k
+        const float yz00 = gy * gz, yz01 = gy * cz, yz10 = cy * gz, yz11 = cy * cz;
+        const float xz00 = gx * gz, xz01 = gx * cz, xz10 = cx * gz, xz11 = cx * cz;
+        const float xy00 = gx * gy, xy01 = gx * cy, xy10 = cx * gy, xy11 = cx * cy;
+        const float ddx = (d[4] - d[0]) * yz00 + (d[5] - d[1]) * yz01 + (d[6] - d[2]) * yz10 + (d[7] - d[3]) * yz11;
+        const float ddy = (d[2] - d[0]) * xz00 + (d[3] - d[1]) * xz01 + (d[6] - d[4]) * xz10 + (d[7] - d[5]) * xz11;
+        const float ddz = (d[1] - d[0]) * xy00 + (d[3] - d[2]) * xy01 + (d[5] - d[4]) * xy10 + (d[7] - d[6]) * xy11;
+        adj_p = mk3(adj_d * ddx, adj_d * ddy, adj_d * ddz);
+    }
+}
+
+}  // namespace dvr

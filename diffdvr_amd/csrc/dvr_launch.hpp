@@ -1,0 +1,56 @@
+// Host-side launcher declarations and the mode -> template dispatch macro.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "dvr_params.hpp"
+
+// Expands CALL<FILTER,TFM,BLEND>ARGS for the runtime (filter, tf, blend) triple.
+#define DVR_DISPATCH_BLEND(F, T, blend, STMT, ARGS)        \
+    switch (blend) {                                        \
+    case 0: STMT<F, T, 0> ARGS; break;                      \
+    case 1: STMT<F, T, 1> ARGS; break;                      \
+    default: break;                                         \
+    }
+#define DVR_DISPATCH_TF(F, tfm, blend, STMT, ARGS)          \
+    switch (tfm) {                                          \
+    case 0: DVR_DISPATCH_BLEND(F, 0, blend, STMT, ARGS) break; \
+    case 1: DVR_DISPATCH_BLEND(F, 1, blend, STMT, ARGS) break; \
+    case 2: DVR_DISPATCH_BLEND(F, 2, blend, STMT, ARGS) break; \
+    case 3: DVR_DISPATCH_BLEND(F, 3, blend, STMT, ARGS) break; \
+    default: break;                                         \
+    }
+#define DVR_DISPATCH3(filter, tfm, blend, STMT, ARGS)       \
+    switch (filter) {                                       \
+    case 0: DVR_DISPATCH_TF(0, tfm, blend, STMT, ARGS) break; \
+    case 1: DVR_DISPATCH_TF(1, tfm, blend, STMT, ARGS) break; \
+    default: break;                                         \
+    }
+
+namespace dvr {
+
+// dvr_kernels_plain.hip
+hipError_t launch_forward_plain(const RenderParams& P, int filter, int tfm, int blend, hipStream_t s);
+hipError_t launch_adjoint_plain(const RenderParams& P, const AdjointParams& A, int filter, int tfm, int blend,
+                                hipStream_t s);
+
+// dvr_kernels_tiled.hip
+// Number of 16x16 tiles of the B x H x W image batch.
+int tiled_num_tiles(const RenderParams& P);
+hipError_t launch_forward_tiled(const RenderParams& P, const CkptParams& CK, int filter, int tfm, int blend,
+                                hipStream_t s);
+// Checkpointed adjoint.  `fused` = run the forward march in the same kernel,
+// write colour/termination to P.color/P.term and form the adjoint colour from
+// the loss L; otherwise A.fcol/A.fterm/A.acol are inputs.  CK.buf holds
+// either checkpoints written by launch_forward_tiled (ckptFromForward) or
+// scratch of tiled_num_tiles()*CK.cap*kBlock floats.
+hipError_t launch_adjoint_tiled(const RenderParams& P, const AdjointParams& A, const CkptParams& CK,
+                                const LossParams& L, bool fused, bool ckptFromForward, int filter, int tfm,
+                                int blend, hipStream_t s);
+
+// dvr_kernels_aux.hip
+hipError_t launch_generate_rays(const float* viewport, int vSb, int vSr, int vSc, int B, int H, int W, float fovY,
+                                float* rs, const int rsS[4], float* rd, const int rdS[4], hipStream_t s);
+hipError_t launch_generate_rays_adjoint(const float* viewport, int vSb, int vSr, int vSc, int B, int H, int W,
+                                        float fovY, const float* ars, const int arsS[4], const float* ard,
+                                        const int ardS[4], float* out, int oSb, int oSr, int oSc, hipStream_t s);
+
+}  // namespace dvr

@@ -1,0 +1,76 @@
+// Kernel parameter blocks shared by the host dispatch and the HIP kernels.
+// Element strides are int32 (the reference's accessors are 32-bit too,
+// renderer/renderer_tensor.cuh:122-269); the host validates that every
+// offset fits.  A stride of 0 on a size-1 batch dimension broadcasts it.
+#pragma once
+#include <stdint.h>
+
+namespace dvr {
+
+constexpr int kTileW = 16;        // pixels per workgroup tile (x)
+constexpr int kTileH = 16;        // pixels per workgroup tile (y)
+constexpr int kBlock = 256;       // threads per workgroup = 4 waves of 8x8 rays
+constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kMaxTfLds = 4096;   // max R*C floats of a TF staged in LDS by the tiled kernels
+constexpr int kMaxSamples = 1 << 24;  // hard cap on samples per ray (termination guarantee)
+
+struct RenderParams {
+    int W, H, B;
+    // volume (B) x X x Y x Z
+    const float* vol; int X, Y, Z; int vsb, vsx, vsy, vsz;
+    // bounding box; pointer NULL -> uniform value
+    const float* bmin; int bminSb, bminSc; float bminU[3];
+    const float* bsize; int bsizeSb, bsizeSc; float bsizeU[3];
+    // camera
+    int camMode;
+    const float* rs; int rsSb, rsSy, rsSx, rsSc;
+    const float* rd; int rdSb, rdSy, rdSx, rdSc;
+    const float* cm; int cmSb, cmSr, cmSc; float cmU[16];
+    float fovY;
+    // step size (B) x (H) x (W); pointer NULL -> uniform value
+    const float* st; int stSb, stSy, stSx; float stU;
+    // transfer function (B) x R x C
+    const float* tf; int R, C; int tfSb, tfSr, tfSc;
+    float earlyOut;
+    // forward outputs
+    float* color; int colSb, colSy, colSx, colSc;
+    int* term; int termSb, termSy, termSx;
+};
+
+struct AdjointParams {
+    // forward outputs used as inputs
+    const float* fcol; int fcSb, fcSy, fcSx, fcSc;
+    const int* fterm; int ftSb, ftSy, ftSx;
+    // adjoint of the output colour
+    const float* acol; int acSb, acSy, acSx, acSc;
+    // adjoint outputs (NULL = not requested)
+    float* avol; int avSb, avSx, avSy, avSz;
+    float* ast; int astSb, astSy, astSx; int stepBroadcast;
+    float* ars; int arsSb, arsSy, arsSx, arsSc;
+    float* ard; int ardSb, ardSy, ardSx, ardSc; int camBroadcast;
+    float* atf; int atfSb, atfSr, atfSc;
+    int tfDelayed;
+};
+
+// Alpha checkpoints: ckpt c (0-based) of a ray holds the accumulated alpha
+// after (c+1)*K samples.  Layout [tile][c][thread] so one wave stores 256 B.
+struct CkptParams {
+    float* buf;   // NULL = none
+    int K;        // checkpoint interval
+    int cap;      // checkpoints per ray
+};
+
+// Per-pixel loss whose gradient the fused forward+adjoint kernel forms in
+// registers between the forward and the reverse march.
+#define DVR_LOSS_NONE 0   // adjoint colour read from AdjointParams::acol
+#define DVR_LOSS_L1 1     // mean |color - target|   (torch.nn.L1Loss)
+#define DVR_LOSS_L2 2     // mean (color - target)^2 (torch.nn.MSELoss)
+
+struct LossParams {
+    int kind;
+    const float* target; int tgSb, tgSy, tgSx, tgSc;
+    float scale;          // 1 / number of loss elements
+    float* lossOut;       // optional scalar accumulator (atomic), may be NULL
+};
+
+}  // namespace dvr

@@ -1,0 +1,471 @@
+// Tiled kernels for gfx950: each workgroup renders a 16x16 pixel tile of one
+// view as four wave64s of 8x8 rays, so one wave's trilinear gathers touch a
+// compact bundle of nearby cells.  Tiles are ordered so that each XCD owns a
+// contiguous band of them (shared L2).  The transfer function is staged in
+// LDS; its gradient is privatised in LDS and flushed once per workgroup.
+//
+// The adjoint is checkpointed: the forward march records the accumulated
+// alpha every K samples; the reverse march recomputes each K-sample segment
+// from its checkpoint into registers and walks it backwards.  No blend step is
+// ever inverted (the reference inverts it, renderer/renderer_blending.cuh:75-77,
+// which loses precision as the accumulated alpha approaches 1).
+//
+// Forward and reverse semantics: renderer/renderer_kernels.cuh:22-90 and
+// :324-444 (step-size adjoint accumulated over all samples, see dvr_device.hpp).
+#pragma once
+#include "dvr_device.hpp"
+#include "dvr_launch.hpp"
+
+namespace dvr {
+
+constexpr int kInvalidCell = -0x40000000;
+
+struct TileXY {
+    int tile, b, x, y;
+};
+
+__device__ __forceinline__ TileXY tile_xy(int tilesX, int tilesPerView, int nTiles)
+{
+    TileXY t;
+    t.tile = xcd_remap(blockIdx.x, nTiles);
+    t.b = t.tile / tilesPerView;
+    const int tl = t.tile - t.b * tilesPerView;
+    const int ty = tl / tilesX, tx = tl - ty * tilesX;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    t.x = tx * kTileW + (wave & 1) * 8 + (lane & 7);
+    t.y = ty * kTileH + (wave >> 1) * 8 + (lane >> 3);
+    return t;
+}
+
+__device__ __forceinline__ void stage_tf(const RenderParams& P, int b, float* s_tf)
+{
+    const float* g = P.tf + b * P.tfSb;
+    const int RC = P.R * P.C;
+    for (int e = threadIdx.x; e < RC; e += blockDim.x) {
+        const int r = e / P.C, c = e - r * P.C;
+        s_tf[e] = g[r * P.tfSr + c * P.tfSc];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward march of one ray.  frozen >= 0 replays exactly that many samples.
+// Writes the accumulated alpha after every K-th sample to ck[c*ckS] for
+// c < cap (ck may be NULL).
+// ---------------------------------------------------------------------------
+template <int FILTER, int TFM, int BLEND>
+__device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const Ray& r, float earlyOut, int frozen,
+                                              f4& acc, int& n, float* ck, int ckS, int K, int cap)
+{
+    Cell c;
+    cell_invalidate(c);
+    acc = mk4(0.f, 0.f, 0.f, 0.f);
+    int countdown = K, cidx = 0;
+    int i = 0;
+    for (; i < kMaxSamples; ++i) {
+        const float t = r.t0 + float(i) * r.step;
+        if (frozen >= 0) {
+            if (i >= frozen) break;
+        } else {
+            if (t > r.t1) break;
+            if (acc.w > earlyOut) break;
+        }
+        const float d = fetch<FILTER>(V, sample_pos(r, t), c);
+        blend<BLEND>(acc, tf_eval<TFM>(T, d), r.step);
+        if (ck) {
+            if (--countdown == 0) {
+                countdown = K;
+                if (cidx < cap) ck[cidx * ckS] = acc.w;
+                ++cidx;
+            }
+        }
+    }
+    n = i;
+}
+
+// ---------------------------------------------------------------------------
+// TF-gradient sink of the tiled adjoint.
+//   identity : two registers per lane, wave-reduced at the end
+//   texture / linear : per-lane cache of the two most recent rows (densities
+//       change slowly along a ray), flushed into a workgroup LDS table with
+//       LDS atomics only when the row changes
+//   gaussian : every sample touches every entry; per-thread private LDS slices
+//       (entry-major, conflict-free) when they fit, else LDS atomics
+// ---------------------------------------------------------------------------
+template <int TFM>
+struct SinkTiled {
+    float* tab;   // workgroup table (R*C) or private slice base (priv)
+    int C;
+    bool priv;
+    int r0, r1;
+    float a0[5], a1[5];
+    float id0, id1;
+
+    __device__ void init(float* table, int C_, bool usePriv)
+    {
+        tab = table; C = C_; priv = usePriv;
+        r0 = r1 = -1;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) { a0[k] = 0.f; a1[k] = 0.f; }
+        id0 = id1 = 0.f;
+    }
+    __device__ void add(int r, int c, float v)
+    {
+        if constexpr (TFM == 0) {
+            if (c == 0) id0 += v; else id1 += v;
+        } else {
+            if (priv) tab[(r * C + c) * kBlock + threadIdx.x] += v;
+            else atomicAdd(&tab[r * C + c], v);
+        }
+    }
+    __device__ void flush_row(int r, float a[5], int ncol)
+    {
+        if (r >= 0)
+            for (int c = 0; c < ncol; ++c)
+                if (a[c] != 0.0f) atomicAdd(&tab[r * C + c], a[c]);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) a[k] = 0.f;
+    }
+    template <bool POS>
+    __device__ void add_pair(int i0, float w0, int i1, float w1, const f4& adj, float p0, float p1)
+    {
+        constexpr int NC = POS ? 5 : 4;
+        if (i0 != r0) { flush_row(r0, a0, NC); r0 = i0; }
+        a0[0] += adj.x * w0; a0[1] += adj.y * w0; a0[2] += adj.z * w0; a0[3] += adj.w * w0;
+        if constexpr (POS) a0[4] += p0;
+        if (i1 != r1) { flush_row(r1, a1, NC); r1 = i1; }
+        a1[0] += adj.x * w1; a1[1] += adj.y * w1; a1[2] += adj.z * w1; a1[3] += adj.w * w1;
+        if constexpr (POS) a1[4] += p1;
+    }
+    // called by every thread of the workgroup (uniform control flow)
+    __device__ void finish(int ncol)
+    {
+        if constexpr (TFM == 0) {
+            const float s0 = wave_sum(id0), s1 = wave_sum(id1);
+            if ((threadIdx.x & 63) == 0) {
+                atomicAdd(&tab[0], s0);
+                atomicAdd(&tab[1], s1);
+            }
+        } else if constexpr (TFM == 1 || TFM == 2) {
+            flush_row(r0, a0, ncol);
+            flush_row(r1, a1, ncol);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Reverse march of one ray with alpha checkpoints.  GM is the set of gradient
+// kinds compiled in: bit0 TF, bit1 volume, bit2 position (camera / step);
+// the runtime flags select among them.
+// ---------------------------------------------------------------------------
+template <int FILTER, int TFM, int BLEND, int K, int GM, class Sink>
+__device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const Ray& r, int n, f4 adj_acc,
+                                              const float* ck, int ckS, int cap, Sink& sink, bool wantTf,
+                                              bool wantVol, bool wantPos, float* avol, int avx, int avy, int avz,
+                                              f3& adj_start, f3& adj_dir, float& adj_step)
+{
+    constexpr bool CT_TF = (GM & 1) != 0, CT_VOL = (GM & 2) != 0, CT_POS = (GM & 4) != 0;
+    wantTf = CT_TF && wantTf;
+    wantVol = CT_VOL && wantVol;
+    wantPos = CT_POS && wantPos;
+    float dv[K], ain[K];
+    // reverse-walk cell state for the volume / position adjoint
+    int rcx = kInvalidCell, rcy = 0, rcz = 0;
+    float rw[8], rd[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { rw[k] = 0.f; rd[k] = 0.f; }
+    int nLast = -1;
+    float nAcc = 0.0f;
+
+    auto flush_cell = [&]() {
+        if (rcx != kInvalidCell) {
+            const int lx = iclamp(rcx, 0, V.X - 1), hx = iclamp(rcx + 1, 0, V.X - 1);
+            const int ly = iclamp(rcy, 0, V.Y - 1), hy = iclamp(rcy + 1, 0, V.Y - 1);
+            const int lz = iclamp(rcz, 0, V.Z - 1), hz = iclamp(rcz + 1, 0, V.Z - 1);
+            const int xs0 = lx * avx, xs1 = hx * avx, ys0 = ly * avy, ys1 = hy * avy, zs0 = lz * avz, zs1 = hz * avz;
+            if (rw[0] != 0.f) atomicAdd(avol + xs0 + ys0 + zs0, rw[0]);
+            if (rw[1] != 0.f) atomicAdd(avol + xs0 + ys0 + zs1, rw[1]);
+            if (rw[2] != 0.f) atomicAdd(avol + xs0 + ys1 + zs0, rw[2]);
+            if (rw[3] != 0.f) atomicAdd(avol + xs0 + ys1 + zs1, rw[3]);
+            if (rw[4] != 0.f) atomicAdd(avol + xs1 + ys0 + zs0, rw[4]);
+            if (rw[5] != 0.f) atomicAdd(avol + xs1 + ys0 + zs1, rw[5]);
+            if (rw[6] != 0.f) atomicAdd(avol + xs1 + ys1 + zs0, rw[6]);
+            if (rw[7] != 0.f) atomicAdd(avol + xs1 + ys1 + zs1, rw[7]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) rw[k] = 0.f;
+    };
+
+    const int nseg = (n + K - 1) / K;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int i0 = seg * K;
+        const int cnt = min(K, n - i0);
+        // accumulated alpha before sample i0
+        float A = 0.0f;
+        {
+            const int c = min(seg, cap);
+            if (c > 0) A = ck[(c - 1) * ckS];
+            Cell cc;
+            cell_invalidate(cc);
+            for (int i = c * K; i < i0; ++i) {  // only when the ray outgrew the checkpoint capacity
+                const float t = r.t0 + float(i) * r.step;
+                const float d = fetch<FILTER>(V, sample_pos(r, t), cc);
+                A = A + (1.0f - A) * blend_alpha<BLEND>(tf_eval<TFM>(T, d).w, r.step);
+            }
+        }
+        // recompute the segment forward, keep density and alpha-before per sample
+        {
+            Cell cc;
+            cell_invalidate(cc);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (j < cnt) {
+                    const float t = r.t0 + float(i0 + j) * r.step;
+                    dv[j] = fetch<FILTER>(V, sample_pos(r, t), cc);
+                    ain[j] = A;
+                    A = A + (1.0f - A) * blend_alpha<BLEND>(tf_eval<TFM>(T, dv[j]).w, r.step);
+                }
+            }
+        }
+        // walk the segment backwards
+#pragma unroll
+        for (int j = K - 1; j >= 0; --j) {
+            if (j < cnt) {
+                const int i = i0 + j;
+                const float t = r.t0 + float(i) * r.step;
+                const f4 s = tf_eval<TFM>(T, dv[j]);
+                f4 adj_s;
+                float adj_st;
+                blend_adjoint<BLEND>(ain[j], s, r.step, adj_acc, adj_s, adj_st);
+                float adj_d;
+                if (wantTf) adj_d = tf_adjoint<TFM, CT_TF>(T, dv[j], adj_s, sink);
+                else {
+                    SinkNone none;
+                    adj_d = tf_adjoint<TFM, false>(T, dv[j], adj_s, none);
+                }
+                if constexpr (CT_VOL || CT_POS) {
+                    if (wantVol || wantPos) {
+                        const f3 p = sample_pos(r, t);
+                        f3 adj_p = mk3(0.f, 0.f, 0.f);
+                        if constexpr (FILTER == 0) {
+                            if (wantVol) {
+                                int gx, gy, gz;
+                                nearest_index(p, V.X, V.Y, V.Z, gx, gy, gz);
+                                const int goff = gx * avx + gy * avy + gz * avz;
+                                if (goff != nLast) {
+                                    if (nLast >= 0 && nAcc != 0.f) atomicAdd(avol + nLast, nAcc);
+                                    nLast = goff;
+                                    nAcc = 0.f;
+                                }
+                                nAcc += adj_d;
+                            }
+                        } else {
+                            const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
+                            if (ix != rcx || iy != rcy || iz != rcz) {
+                                if (wantVol) flush_cell();
+                                rcx = ix; rcy = iy; rcz = iz;
+                                if (wantPos) cell_load(V, ix, iy, iz, rd);
+                            }
+                            const float fx = p.x - float(ix), fy = p.y - float(iy), fz = p.z - float(iz);
+                            float w[8];
+                            if (wantPos) tri_adjoint<true>(rd, fx, fy, fz, adj_d, w, adj_p);
+                            else tri_adjoint<false>(rd, fx, fy, fz, adj_d, w, adj_p);
+                            if (wantVol) {
+#pragma unroll
+                                for (int k = 0; k < 8; ++k) rw[k] += w[k];
+                            }
+                        }
+                        if (wantPos) {
+                            const f3 adj_w = mul3(adj_p, r.invVoxel);
+                            adj_start = add3(adj_start, adj_w);
+                            adj_dir = add3(adj_dir, scl3(adj_w, t));
+                            adj_step += dot3(adj_w, r.dir) * float(i);
+                        }
+                    }
+                }
+                adj_step += adj_st;
+            }
+        }
+    }
+    if constexpr (CT_VOL) {
+        if (wantVol) {
+            if constexpr (FILTER == 0) {
+                if (nLast >= 0 && nAcc != 0.f) atomicAdd(avol + nLast, nAcc);
+            } else {
+                flush_cell();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <int FILTER, int TFM, int BLEND>
+__global__ __launch_bounds__(kBlock) void k_forward_tiled(RenderParams P, CkptParams CK, int tilesX, int tilesPerView,
+                                                          int nTiles)
+{
+    extern __shared__ float lds[];
+    const TileXY tc = tile_xy(tilesX, tilesPerView, nTiles);
+    stage_tf(P, tc.b, lds);
+    __syncthreads();
+    if (tc.x >= P.W || tc.y >= P.H) return;
+    const Ray r = setup_ray(P, tc.x, tc.y, tc.b);
+    const Vol V = make_vol(P, tc.b);
+    const Tf T{lds, P.R, P.C, P.C, 1};
+    float* ck = CK.buf ? CK.buf + (size_t(tc.tile) * CK.cap) * kBlock + threadIdx.x : nullptr;
+    f4 acc;
+    int n;
+    march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, CK.K, CK.cap);
+    float* o = P.color + tc.b * P.colSb + tc.y * P.colSy + tc.x * P.colSx;
+    o[0] = acc.x; o[P.colSc] = acc.y; o[2 * P.colSc] = acc.z; o[3 * P.colSc] = acc.w;
+    P.term[tc.b * P.termSb + tc.y * P.termSy + tc.x * P.termSx] = n;
+}
+
+template <int FILTER, int TFM, int BLEND, int K, int GM>
+__global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, AdjointParams A, CkptParams CK, LossParams L,
+                                                          int fused, int ckFromFwd, int usePriv, int tilesX,
+                                                          int tilesPerView, int nTiles)
+{
+    extern __shared__ float lds[];
+    const int RC = P.R * P.C;
+    float* s_tf = lds;
+    float* s_g = lds + RC;
+    const TileXY tc = tile_xy(tilesX, tilesPerView, nTiles);
+    const bool wantTf = A.atf != nullptr;
+    stage_tf(P, tc.b, s_tf);
+    if (wantTf) {
+        const int ng = usePriv ? RC * kBlock : RC;
+        for (int e = threadIdx.x; e < ng; e += kBlock) s_g[e] = 0.0f;
+    }
+    __syncthreads();
+
+    SinkTiled<TFM> sink;
+    sink.init(s_g, P.C, usePriv != 0);
+    float lossLocal = 0.0f;
+    const bool valid = tc.x < P.W && tc.y < P.H;
+    if (valid) {
+        const int b = tc.b, x = tc.x, y = tc.y;
+        const Ray r = setup_ray(P, x, y, b);
+        const Vol V = make_vol(P, b);
+        const Tf T{s_tf, P.R, P.C, P.C, 1};
+        float* ck = CK.buf + (size_t(tc.tile) * CK.cap) * kBlock + threadIdx.x;
+        int n;
+        f4 adj_acc;
+        if (fused) {
+            f4 acc;
+            march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, K, CK.cap);
+            float* o = P.color + b * P.colSb + y * P.colSy + x * P.colSx;
+            o[0] = acc.x; o[P.colSc] = acc.y; o[2 * P.colSc] = acc.z; o[3 * P.colSc] = acc.w;
+            P.term[b * P.termSb + y * P.termSy + x * P.termSx] = n;
+            const float* tg = L.target + b * L.tgSb + y * L.tgSy + x * L.tgSx;
+            const float d0 = acc.x - tg[0], d1 = acc.y - tg[L.tgSc], d2 = acc.z - tg[2 * L.tgSc],
+                        d3 = acc.w - tg[3 * L.tgSc];
+            if (L.kind == DVR_LOSS_L1) {
+                auto sgn = [](float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); };
+                adj_acc = mk4(sgn(d0) * L.scale, sgn(d1) * L.scale, sgn(d2) * L.scale, sgn(d3) * L.scale);
+                lossLocal = (fabsf(d0) + fabsf(d1) + fabsf(d2) + fabsf(d3)) * L.scale;
+            } else {
+                adj_acc = mk4(2.f * d0 * L.scale, 2.f * d1 * L.scale, 2.f * d2 * L.scale, 2.f * d3 * L.scale);
+                lossLocal = (d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3) * L.scale;
+            }
+        } else {
+            n = min(A.fterm[b * A.ftSb + y * A.ftSy + x * A.ftSx], kMaxSamples);
+            const float* ac = A.acol + b * A.acSb + y * A.acSy + x * A.acSx;
+            adj_acc = mk4(ac[0], ac[A.acSc], ac[2 * A.acSc], ac[3 * A.acSc]);
+            if (!ckFromFwd) {
+                f4 acc;
+                int n2;
+                march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, n, acc, n2, ck, kBlock, K, CK.cap);
+            }
+        }
+        f3 adj_start = mk3(0.f, 0.f, 0.f), adj_dir = mk3(0.f, 0.f, 0.f);
+        float adj_step = 0.0f;
+        const bool wantVol = A.avol != nullptr;
+        const bool wantPos = A.ars != nullptr || A.ast != nullptr;
+        march_reverse<FILTER, TFM, BLEND, K, GM>(V, T, r, n, adj_acc, ck, kBlock, CK.cap, sink, wantTf, wantVol,
+                                                 wantPos, wantVol ? A.avol + b * A.avSb : nullptr, A.avSx, A.avSy,
+                                                 A.avSz, adj_start, adj_dir, adj_step);
+        if (A.ast) {
+            float* o = A.ast + b * A.astSb + y * A.astSy + x * A.astSx;
+            if (A.stepBroadcast) atomicAdd(o, adj_step);
+            else *o = adj_step;
+        }
+        if (A.ars) {
+            float* os = A.ars + b * A.arsSb + y * A.arsSy + x * A.arsSx;
+            float* od = A.ard + b * A.ardSb + y * A.ardSy + x * A.ardSx;
+            if (A.camBroadcast) {
+                atomicAdd(os, adj_start.x); atomicAdd(os + A.arsSc, adj_start.y); atomicAdd(os + 2 * A.arsSc, adj_start.z);
+                atomicAdd(od, adj_dir.x); atomicAdd(od + A.ardSc, adj_dir.y); atomicAdd(od + 2 * A.ardSc, adj_dir.z);
+            } else {
+                os[0] = adj_start.x; os[A.arsSc] = adj_start.y; os[2 * A.arsSc] = adj_start.z;
+                od[0] = adj_dir.x; od[A.ardSc] = adj_dir.y; od[2 * A.ardSc] = adj_dir.z;
+            }
+        }
+    }
+    if (wantTf) sink.finish(TFM == 2 ? 5 : 4);
+    if (fused && L.lossOut) {
+        const float s = wave_sum(lossLocal);
+        if ((threadIdx.x & 63) == 0) atomicAdd(L.lossOut, s);
+    }
+    if (wantTf) {
+        __syncthreads();
+        float* g = A.atf + tc.b * A.atfSb;
+        for (int e = threadIdx.x; e < RC; e += kBlock) {
+            float v;
+            if (usePriv) {
+                v = 0.0f;
+                for (int t = 0; t < kBlock; ++t) v += s_g[e * kBlock + t];
+            } else {
+                v = s_g[e];
+            }
+            const int rr = e / P.C, cc = e - rr * P.C;
+            if (v != 0.0f) atomicAdd(g + rr * A.atfSr + cc * A.atfSc, v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch of one instance
+// ---------------------------------------------------------------------------
+template <int FILTER, int TFM, int BLEND, int K, int GM>
+static hipError_t adj_tiled_k(const RenderParams& P, const AdjointParams& A, const CkptParams& CK, const LossParams& L,
+                              bool fused, bool ckFromFwd, hipStream_t s)
+{
+    const int tilesX = (P.W + kTileW - 1) / kTileW, tilesY = (P.H + kTileH - 1) / kTileH;
+    const int nTiles = tilesX * tilesY * P.B;
+    if (nTiles == 0) return hipSuccess;
+    const int RC = P.R * P.C;
+    const bool usePriv = (TFM == 3) && A.atf && RC <= 48;  // private slices <= 48 KiB of LDS
+    const size_t lds = sizeof(float) * (size_t(RC) + (A.atf ? (usePriv ? size_t(RC) * kBlock : size_t(RC)) : 0));
+    hipLaunchKernelGGL((k_adjoint_tiled<FILTER, TFM, BLEND, K, GM>), dim3(nTiles), dim3(kBlock), lds, s, P, A, CK, L,
+                       fused ? 1 : 0, ckFromFwd ? 1 : 0, usePriv ? 1 : 0, tilesX, tilesX * tilesY, nTiles);
+    return hipGetLastError();
+}
+
+// gradient kinds compiled into the instance: TF only, volume (+TF), or everything
+inline int tiled_grad_mask(const AdjointParams& A)
+{
+    const bool pos = A.ars || A.ast;
+    return pos ? 7 : (A.avol ? 3 : 1);
+}
+
+template <int FILTER, int BLEND, int K>
+static hipError_t adj_tiled_fbk(const RenderParams& P, const AdjointParams& A, const CkptParams& CK,
+                                const LossParams& L, bool fused, bool ckFromFwd, int tfm, hipStream_t s)
+{
+    const int gm = tiled_grad_mask(A);
+#define DVR_GM_CASES(TV)                                                                              \
+    if (gm == 1) return adj_tiled_k<FILTER, TV, BLEND, K, 1>(P, A, CK, L, fused, ckFromFwd, s);      \
+    if (gm == 3) return adj_tiled_k<FILTER, TV, BLEND, K, 3>(P, A, CK, L, fused, ckFromFwd, s);      \
+    return adj_tiled_k<FILTER, TV, BLEND, K, 7>(P, A, CK, L, fused, ckFromFwd, s);
+    switch (tfm) {
+    case 0: { DVR_GM_CASES(0) }
+    case 1: { DVR_GM_CASES(1) }
+    case 2: { DVR_GM_CASES(2) }
+    case 3: { DVR_GM_CASES(3) }
+    default: return hipErrorInvalidValue;
+    }
+#undef DVR_GM_CASES
+}
+
+}  // namespace dvr

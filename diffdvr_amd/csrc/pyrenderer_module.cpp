@@ -1,0 +1,489 @@
+// pybind11 module `_pyrenderer`: the pyrenderer API surface of DiffDVR
+// (bindings/bindings.cpp:132-688) on top of the C-ABI in include/dvr.h.
+//
+// Tensors are taken duck-typed from PyTorch (data_ptr / shape / stride /
+// dtype / device) and handed to libdvr_hip.so as plain strided views; calls
+// are issued on the caller's current HIP stream.  This module links only the
+// C-ABI library, not libtorch.
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "dvr.h"
+
+namespace py = pybind11;
+
+namespace {
+
+struct real3 { float x = 0, y = 0, z = 0; };
+struct real4 { float x = 0, y = 0, z = 0, w = 0; };
+struct int2 { int x = 0, y = 0; };
+struct int3 { int x = 0, y = 0, z = 0; };
+
+enum class VolumeFilterMode { Nearest = 0, Trilinear = 1, Preshaded = 3 };
+enum class CameraMode { RayStartDir = 0, InverseViewMatrix = 1, ReferenceFrame = 2 };
+enum class TFMode { Identity = 0, Texture = 1, Linear = 2, Gaussian = 3, Preshaded = 5, GaussianLog = 6 };
+enum class BlendMode { BeerLambert = 0, Alpha = 1 };
+enum class Orientation { Xp = 0, Xm = 1, Yp = 2, Ym = 3, Zp = 4, Zm = 5 };
+
+struct CameraPerPixelRays {
+    py::object ray_start = py::none(), ray_dir = py::none();
+};
+struct CameraReferenceFrame {
+    py::object viewport = py::none();
+    float fov_y_radians = 0;
+};
+// glm::mat4 stand-in: m[i][j] = buffer[i][j], rows as given (bindings.cpp:330-379)
+struct mat4 {
+    float m[16] = {0};
+};
+
+struct RendererInputs {
+    int2 screen_size;
+    py::object volume = py::none();
+    VolumeFilterMode volume_filter_mode = VolumeFilterMode::Trilinear;
+    py::object box_min = py::cast(real3{});
+    py::object box_size = py::cast(real3{});
+    CameraMode camera_mode = CameraMode::RayStartDir;
+    py::object camera = py::none();
+    py::object step_size = py::float_(0.0);
+    TFMode tf_mode = TFMode::Identity;
+    py::object tf = py::none();
+    BlendMode blend_mode = BlendMode::BeerLambert;
+    float blending_early_out = 1.0f - 1e-5f;
+};
+
+struct RendererOutputs {
+    py::object color = py::none(), terminationIndex = py::none();
+};
+
+struct AdjointOutputs {
+    bool has_volume_derivatives = false;
+    py::object adj_volume = py::none();
+    bool has_stepsize_derivatives = false;
+    py::object adj_stepsize = py::none();
+    bool has_camera_derivatives = false;
+    py::object adj_camera_ray_start = py::none(), adj_camera_ray_dir = py::none();
+    bool has_tf_derivatives = false;
+    bool tf_delayed_accumulation = false;
+    py::object adj_tf = py::none();
+};
+
+struct ForwardDifferencesSettings {
+    int D = 0;
+    int d_stepsize = -1;
+    int3 d_rayStart{-1, -1, -1}, d_rayDir{-1, -1, -1};
+    py::object d_tf = py::none();
+    bool has_tf_derivatives = false;
+};
+
+// ---------------------------------------------------------------------------
+// tensor -> dvr_tensor
+// ---------------------------------------------------------------------------
+struct TorchInfo {
+    py::object torch, cuda_current_stream, cuda_current_device, float32, int32;
+};
+TorchInfo& torch_info()
+{
+    static TorchInfo* t = nullptr;
+    if (!t) {
+        t = new TorchInfo();
+        t->torch = py::module_::import("torch");
+        t->cuda_current_stream = t->torch.attr("cuda").attr("current_stream");
+        t->cuda_current_device = t->torch.attr("cuda").attr("current_device");
+        t->float32 = t->torch.attr("float32");
+        t->int32 = t->torch.attr("int32");
+    }
+    return *t;
+}
+
+dvr_tensor to_tensor(const py::handle& h)
+{
+    dvr_tensor t;
+    std::memset(&t, 0, sizeof(t));
+    if (h.is_none()) return t;
+    if (!py::hasattr(h, "data_ptr") || !py::hasattr(h, "stride"))
+        throw std::invalid_argument("expected a torch.Tensor");
+    TorchInfo& ti = torch_info();
+    py::tuple shape = py::tuple(h.attr("shape"));
+    py::tuple stride = py::tuple(h.attr("stride")());
+    t.ndim = (int)shape.size();
+    if (t.ndim > 6) throw std::invalid_argument("tensors with more than 6 dimensions are not supported");
+    for (int d = 0; d < t.ndim; ++d) {
+        t.sizes[d] = shape[d].cast<int64_t>();
+        t.strides[d] = stride[d].cast<int64_t>();
+    }
+    t.data = reinterpret_cast<void*>(h.attr("data_ptr")().cast<uintptr_t>());
+    py::object dt = h.attr("dtype");
+    t.dtype = dt.is(ti.float32) ? DVR_FLOAT32 : (dt.is(ti.int32) ? DVR_INT32 : -1);
+    py::object dev = h.attr("device");
+    const std::string type = py::str(dev.attr("type"));
+    if (type == "cuda") {
+        py::object idx = dev.attr("index");
+        t.device = idx.is_none() ? ti.cuda_current_device().cast<int>() : idx.cast<int>();
+    } else {
+        t.device = -1;
+    }
+    return t;
+}
+
+void* current_stream(int device)
+{
+    if (device < 0) return nullptr;
+    py::object s = torch_info().cuda_current_stream(device);
+    return reinterpret_cast<void*>(s.attr("cuda_stream").cast<uintptr_t>());
+}
+
+void check(dvr_status s)
+{
+    if (s != DVR_OK) throw std::runtime_error(dvr_last_error());
+}
+
+// Keeps the Python objects behind a dvr_inputs alive for the call.
+struct PackedInputs {
+    dvr_inputs in;
+};
+
+PackedInputs pack_inputs(const RendererInputs& r)
+{
+    PackedInputs p;
+    dvr_inputs& in = p.in;
+    std::memset(&in, 0, sizeof(in));
+    in.screen_width = r.screen_size.x;
+    in.screen_height = r.screen_size.y;
+    in.volume_filter_mode = (int)r.volume_filter_mode;
+    in.volume = to_tensor(r.volume);
+    auto vec3 = [](const py::object& o, dvr_tensor& t, float* v) {
+        if (py::isinstance<real3>(o)) {
+            const real3 x = o.cast<real3>();
+            v[0] = x.x; v[1] = x.y; v[2] = x.z;
+        } else {
+            t = to_tensor(o);
+        }
+    };
+    vec3(r.box_min, in.box_min, in.box_min_value);
+    vec3(r.box_size, in.box_size, in.box_size_value);
+    in.camera_mode = (int)r.camera_mode;
+    switch (r.camera_mode) {
+    case CameraMode::RayStartDir:
+        if (!py::isinstance<CameraPerPixelRays>(r.camera))
+            throw std::runtime_error("'CameraRayStartDir' specified in 'cameraMode', but 'camera' does not hold the "
+                                     "first variant 'CameraPerPixelRays'");
+        {
+            const CameraPerPixelRays& c = r.camera.cast<const CameraPerPixelRays&>();
+            in.camera_ray_start = to_tensor(c.ray_start);
+            in.camera_ray_dir = to_tensor(c.ray_dir);
+        }
+        break;
+    case CameraMode::InverseViewMatrix:
+        if (py::isinstance<CameraPerPixelRays>(r.camera) || py::isinstance<CameraReferenceFrame>(r.camera))
+            throw std::runtime_error("'CameraInverseViewMatrix' specified in 'cameraMode', but 'camera' holds neither "
+                                     "a B*4*4 tensor nor a mat4");
+        if (py::isinstance<mat4>(r.camera)) {
+            const mat4& m = r.camera.cast<const mat4&>();
+            std::memcpy(in.camera_matrix_value, m.m, sizeof(m.m));
+        } else {
+            in.camera_matrix = to_tensor(r.camera);
+        }
+        break;
+    case CameraMode::ReferenceFrame:
+        if (!py::isinstance<CameraReferenceFrame>(r.camera))
+            throw std::runtime_error("'CameraReferenceFrame' specified in 'cameraMode', but 'camera' does not hold the "
+                                     "fourth variant 'CameraReferenceFrame'");
+        {
+            const CameraReferenceFrame& c = r.camera.cast<const CameraReferenceFrame&>();
+            in.camera_matrix = to_tensor(c.viewport);
+            in.camera_fov_y_radians = c.fov_y_radians;
+        }
+        break;
+    }
+    if (py::isinstance<py::float_>(r.step_size) || py::isinstance<py::int_>(r.step_size))
+        in.step_size_value = r.step_size.cast<float>();
+    else
+        in.step_size = to_tensor(r.step_size);
+    in.tf_mode = (int)r.tf_mode;
+    in.tf = to_tensor(r.tf);
+    in.blend_mode = (int)r.blend_mode;
+    in.blending_early_out = r.blending_early_out;
+    if (r.volume_filter_mode == VolumeFilterMode::Preshaded || r.tf_mode == TFMode::Preshaded ||
+        r.tf_mode == TFMode::GaussianLog)
+        throw std::runtime_error("Preshaded / GaussianLog modes are not available in this build");
+    return p;
+}
+
+dvr_outputs pack_outputs(const RendererOutputs& o)
+{
+    dvr_outputs out;
+    std::memset(&out, 0, sizeof(out));
+    out.color = to_tensor(o.color);
+    out.termination_index = to_tensor(o.terminationIndex);
+    return out;
+}
+
+dvr_adjoint_outputs pack_adjoint(const AdjointOutputs& a)
+{
+    dvr_adjoint_outputs o;
+    std::memset(&o, 0, sizeof(o));
+    o.has_volume_derivatives = a.has_volume_derivatives;
+    if (a.has_volume_derivatives) o.adj_volume = to_tensor(a.adj_volume);
+    o.has_stepsize_derivatives = a.has_stepsize_derivatives;
+    if (a.has_stepsize_derivatives) o.adj_stepsize = to_tensor(a.adj_stepsize);
+    o.has_camera_derivatives = a.has_camera_derivatives;
+    if (a.has_camera_derivatives) {
+        o.adj_camera_ray_start = to_tensor(a.adj_camera_ray_start);
+        o.adj_camera_ray_dir = to_tensor(a.adj_camera_ray_dir);
+    }
+    o.has_tf_derivatives = a.has_tf_derivatives;
+    if (a.has_tf_derivatives) o.adj_tf = to_tensor(a.adj_tf);
+    o.tf_delayed_accumulation = a.tf_delayed_accumulation;
+    return o;
+}
+
+int parse_path(const std::string& s)
+{
+    if (s == "auto") return DVR_PATH_AUTO;
+    if (s == "plain") return DVR_PATH_PLAIN;
+    if (s == "tiled") return DVR_PATH_TILED;
+    throw std::invalid_argument("path must be 'auto', 'plain' or 'tiled'");
+}
+int parse_method(const std::string& s)
+{
+    if (s == "auto") return DVR_ADJOINT_AUTO;
+    if (s == "inversion") return DVR_ADJOINT_INVERSION;
+    if (s == "checkpoint") return DVR_ADJOINT_CHECKPOINT;
+    throw std::invalid_argument("method must be 'auto', 'inversion' or 'checkpoint'");
+}
+
+struct GpuTimer {
+    dvr_timer* t = nullptr;
+    GpuTimer() { check(dvr_timer_create(&t)); }
+    ~GpuTimer() { dvr_timer_destroy(t); }
+    void start() { check(dvr_timer_start(t, current_stream(torch_info().cuda_current_device().cast<int>()))); }
+    void stop() { check(dvr_timer_stop(t, current_stream(torch_info().cuda_current_device().cast<int>()))); }
+    float elapsed_ms()
+    {
+        float ms = 0;
+        check(dvr_timer_elapsed_ms(t, &ms));
+        return ms;
+    }
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_pyrenderer, m)
+{
+    m.doc() = "MI355X (gfx950) differentiable direct volume renderer: pyrenderer API over libdvr_hip";
+    m.def("use_double_precision", []() { return dvr_use_double_precision(); },
+          "Returns 0: this library computes in single precision.");
+    m.def("version", []() { return std::string(dvr_version()); });
+    m.def("sync", []() { check(dvr_sync()); });
+    m.def("cleanup", []() {});
+    m.def("reload_kernels", []() {});
+    m.def("set_cuda_sync_mode", [](bool) {});
+    m.def("set_cuda_debug_mode", [](bool) {});
+    m.def("set_cuda_cache_dir", [](const std::string&) {});
+    m.def("disable_cuda_cache", []() {});
+
+    py::class_<real3>(m, "real3")
+        .def(py::init<>())
+        .def(py::init([](float x, float y, float z) { return real3{x, y, z}; }))
+        .def_readwrite("x", &real3::x).def_readwrite("y", &real3::y).def_readwrite("z", &real3::z)
+        .def("__str__", [](const real3& v) { return "(" + std::to_string(v.x) + ", " + std::to_string(v.y) + ", " + std::to_string(v.z) + ")"; });
+    py::class_<real4>(m, "real4")
+        .def(py::init<>())
+        .def(py::init([](float x, float y, float z, float w) { return real4{x, y, z, w}; }))
+        .def_readwrite("x", &real4::x).def_readwrite("y", &real4::y).def_readwrite("z", &real4::z)
+        .def_readwrite("w", &real4::w)
+        .def("__str__", [](const real4& v) {
+            return "(" + std::to_string(v.x) + ", " + std::to_string(v.y) + ", " + std::to_string(v.z) + ", " + std::to_string(v.w) + ")";
+        });
+    py::class_<int2>(m, "int2")
+        .def(py::init<>())
+        .def(py::init([](int x, int y) { return int2{x, y}; }))
+        .def_readwrite("x", &int2::x).def_readwrite("y", &int2::y)
+        .def("__str__", [](const int2& v) { return "(" + std::to_string(v.x) + ", " + std::to_string(v.y) + ")"; });
+    py::class_<int3>(m, "int3")
+        .def(py::init<>())
+        .def(py::init([](int x, int y, int z) { return int3{x, y, z}; }))
+        .def_readwrite("x", &int3::x).def_readwrite("y", &int3::y).def_readwrite("z", &int3::z)
+        .def("__str__", [](const int3& v) { return "(" + std::to_string(v.x) + ", " + std::to_string(v.y) + ", " + std::to_string(v.z) + ")"; });
+
+    py::enum_<VolumeFilterMode>(m, "VolumeFilterMode")
+        .value("Nearest", VolumeFilterMode::Nearest)
+        .value("Trilinear", VolumeFilterMode::Trilinear)
+        .value("Preshaded", VolumeFilterMode::Preshaded);
+    py::enum_<CameraMode>(m, "CameraMode")
+        .value("RayStartDir", CameraMode::RayStartDir)
+        .value("InverseViewMatrix", CameraMode::InverseViewMatrix)
+        .value("ReferenceFrame", CameraMode::ReferenceFrame);
+    py::enum_<TFMode>(m, "TFMode")
+        .value("Identity", TFMode::Identity)
+        .value("Texture", TFMode::Texture)
+        .value("Linear", TFMode::Linear)
+        .value("Gaussian", TFMode::Gaussian)
+        .value("GaussianLog", TFMode::GaussianLog)
+        .value("Preshaded", TFMode::Preshaded);
+    py::enum_<BlendMode>(m, "BlendMode")
+        .value("BeerLambert", BlendMode::BeerLambert)
+        .value("Alpha", BlendMode::Alpha);
+    py::enum_<Orientation>(m, "Orientation")
+        .value("Xp", Orientation::Xp).value("Xm", Orientation::Xm)
+        .value("Yp", Orientation::Yp).value("Ym", Orientation::Ym)
+        .value("Zp", Orientation::Zp).value("Zm", Orientation::Zm);
+
+    py::class_<CameraPerPixelRays>(m, "CameraPerPixelRays")
+        .def(py::init<>())
+        .def(py::init([](py::object s, py::object d) { return CameraPerPixelRays{s, d}; }))
+        .def_readwrite("ray_start", &CameraPerPixelRays::ray_start, "Tensor of ray starting positions of shape B*H*W*3")
+        .def_readwrite("ray_dir", &CameraPerPixelRays::ray_dir, "Tensor of ray direction of shape B*H*W*3");
+    py::class_<CameraReferenceFrame>(m, "CameraReferenceFrame")
+        .def(py::init<>())
+        .def(py::init([](py::object v, float fov) { return CameraReferenceFrame{v, fov}; }))
+        .def_readwrite("viewport", &CameraReferenceFrame::viewport,
+                       "Tensor of the viewport matrix of shape B*3*3 (eye, right, up)")
+        .def_readwrite("fov_y_radians", &CameraReferenceFrame::fov_y_radians, "field-of-view for the y-axis in radians");
+    py::class_<mat4>(m, "mat4")
+        .def(py::init([](py::object b) {
+            mat4 r;
+            py::sequence rows = py::reinterpret_borrow<py::sequence>(b.attr("tolist")());
+            if (rows.size() != 4) throw std::runtime_error("Incompatible buffer sizes! Expected a shape of 4x4");
+            for (int i = 0; i < 4; ++i) {
+                py::sequence row = rows[i];
+                if (row.size() != 4) throw std::runtime_error("Incompatible buffer sizes! Expected a shape of 4x4");
+                for (int j = 0; j < 4; ++j) r.m[4 * i + j] = row[j].cast<float>();
+            }
+            return r;
+        }))
+        .def("tolist", [](const mat4& m) {
+            py::list l;
+            for (int i = 0; i < 4; ++i) {
+                py::list row;
+                for (int j = 0; j < 4; ++j) row.append(m.m[4 * i + j]);
+                l.append(row);
+            }
+            return l;
+        });
+
+    py::class_<RendererInputs>(m, "RendererInputs")
+        .def(py::init<>())
+        .def("clone", [](const RendererInputs& a) { return RendererInputs(a); }, "performs a shallow clone")
+        .def_readwrite("screen_size", &RendererInputs::screen_size, "the screen size (width, height) of type int2")
+        .def_readwrite("volume", &RendererInputs::volume, "the volume tensor of shape B*X*Y*Z")
+        .def_readwrite("volume_filter_mode", &RendererInputs::volume_filter_mode)
+        .def_readwrite("box_min", &RendererInputs::box_min, "real3 or float tensor of shape B*3")
+        .def_readwrite("box_size", &RendererInputs::box_size, "real3 or float tensor of shape B*3")
+        .def_readwrite("camera_mode", &RendererInputs::camera_mode)
+        .def_readwrite("camera", &RendererInputs::camera,
+                       "CameraPerPixelRays | B*4*4 tensor or mat4 | CameraReferenceFrame, per camera_mode")
+        .def_readwrite("step_size", &RendererInputs::step_size, "float or float tensor of shape B*H*W")
+        .def_readwrite("tf_mode", &RendererInputs::tf_mode)
+        .def_readwrite("tf", &RendererInputs::tf, "transfer function tensor of shape B*R*C")
+        .def_readwrite("blend_mode", &RendererInputs::blend_mode)
+        .def_readwrite("blending_early_out", &RendererInputs::blending_early_out);
+
+    py::class_<RendererOutputs>(m, "RendererOutputs")
+        .def(py::init<>())
+        .def(py::init([](py::object c, py::object t) { return RendererOutputs{c, t}; }))
+        .def_readwrite("color", &RendererOutputs::color)
+        .def_readwrite("terminationIndex", &RendererOutputs::terminationIndex);
+
+    py::class_<ForwardDifferencesSettings>(m, "ForwardDifferencesSettings")
+        .def(py::init<>())
+        .def_readwrite("D", &ForwardDifferencesSettings::D)
+        .def_readwrite("d_stepsize", &ForwardDifferencesSettings::d_stepsize)
+        .def_readwrite("d_rayStart", &ForwardDifferencesSettings::d_rayStart)
+        .def_readwrite("d_rayDir", &ForwardDifferencesSettings::d_rayDir)
+        .def_readwrite("d_tf", &ForwardDifferencesSettings::d_tf)
+        .def_readwrite("has_tf_derivatives", &ForwardDifferencesSettings::has_tf_derivatives);
+
+    py::class_<AdjointOutputs>(m, "AdjointOutputs")
+        .def(py::init<>())
+        .def_readwrite("has_volume_derivatives", &AdjointOutputs::has_volume_derivatives)
+        .def_readwrite("adj_volume", &AdjointOutputs::adj_volume)
+        .def_readwrite("has_stepsize_derivatives", &AdjointOutputs::has_stepsize_derivatives)
+        .def_readwrite("adj_stepsize", &AdjointOutputs::adj_stepsize)
+        .def_readwrite("has_camera_derivatives", &AdjointOutputs::has_camera_derivatives)
+        .def_readwrite("adj_camera_ray_start", &AdjointOutputs::adj_camera_ray_start)
+        .def_readwrite("adj_camera_ray_dir", &AdjointOutputs::adj_camera_ray_dir)
+        .def_readwrite("has_tf_derivatives", &AdjointOutputs::has_tf_derivatives)
+        .def_readwrite("tf_delayed_accumulation", &AdjointOutputs::tf_delayed_accumulation)
+        .def_readwrite("adj_tf", &AdjointOutputs::adj_tf);
+
+    py::class_<GpuTimer>(m, "GpuTimer")
+        .def(py::init<>())
+        .def("start", &GpuTimer::start)
+        .def("stop", &GpuTimer::stop)
+        .def("elapsed_ms", &GpuTimer::elapsed_ms);
+
+    m.def("render_forward",
+          [](const RendererInputs& inputs, RendererOutputs& outputs, const std::string& path, int checkpoint_interval,
+             py::object checkpoints) {
+              PackedInputs p = pack_inputs(inputs);
+              dvr_outputs o = pack_outputs(outputs);
+              dvr_forward_options opt;
+              std::memset(&opt, 0, sizeof(opt));
+              opt.path = parse_path(path);
+              opt.checkpoint_interval = checkpoint_interval;
+              opt.checkpoints = to_tensor(checkpoints);
+              check(dvr_render_forward_ex(&p.in, &o, &opt, current_stream(p.in.volume.device)));
+          },
+          py::arg("inputs"), py::arg("outputs"), py::arg("path") = "auto", py::arg("checkpoint_interval") = 0,
+          py::arg("checkpoints") = py::none());
+    m.def("render_adjoint",
+          [](const RendererInputs& inputs, const RendererOutputs& fwd, py::object adj_color, AdjointOutputs& adj,
+             const std::string& method, const std::string& path, int checkpoint_interval, py::object checkpoints) {
+              PackedInputs p = pack_inputs(inputs);
+              dvr_outputs o = pack_outputs(fwd);
+              dvr_tensor ac = to_tensor(adj_color);
+              dvr_adjoint_outputs ao = pack_adjoint(adj);
+              dvr_adjoint_options opt;
+              std::memset(&opt, 0, sizeof(opt));
+              opt.method = parse_method(method);
+              opt.path = parse_path(path);
+              opt.checkpoint_interval = checkpoint_interval;
+              opt.checkpoints = to_tensor(checkpoints);
+              check(dvr_render_adjoint_ex(&p.in, &o, &ac, &ao, &opt, current_stream(p.in.volume.device)));
+          },
+          py::arg("inputs"), py::arg("outputs_from_forward"), py::arg("adj_color"), py::arg("adj_outputs"),
+          py::arg("method") = "auto", py::arg("path") = "auto", py::arg("checkpoint_interval") = 0,
+          py::arg("checkpoints") = py::none());
+    m.def("render_fused",
+          [](const RendererInputs& inputs, RendererOutputs& outputs, const std::string& loss, py::object target,
+             AdjointOutputs& adj, py::object loss_out, float scale, int checkpoint_interval) {
+              PackedInputs p = pack_inputs(inputs);
+              dvr_outputs o = pack_outputs(outputs);
+              dvr_loss l;
+              std::memset(&l, 0, sizeof(l));
+              if (loss == "l1") l.kind = DVR_LOSS_L1;
+              else if (loss == "l2" || loss == "mse") l.kind = DVR_LOSS_L2;
+              else throw std::invalid_argument("loss must be 'l1' or 'l2'");
+              l.target = to_tensor(target);
+              l.scale = scale;
+              l.loss = to_tensor(loss_out);
+              dvr_adjoint_outputs ao = pack_adjoint(adj);
+              check(dvr_render_fused(&p.in, &o, &l, &ao, checkpoint_interval, current_stream(p.in.volume.device)));
+          },
+          py::arg("inputs"), py::arg("outputs"), py::arg("loss"), py::arg("target"), py::arg("adj_outputs"),
+          py::arg("loss_out") = py::none(), py::arg("scale") = 0.0f, py::arg("checkpoint_interval") = 0);
+    m.def("checkpoint_buffer_elements",
+          [](const RendererInputs& inputs, int K) {
+              PackedInputs p = pack_inputs(inputs);
+              const int64_t n = dvr_checkpoint_buffer_elements(&p.in, K);
+              if (n < 0) throw std::runtime_error(dvr_last_error());
+              return n;
+          },
+          py::arg("inputs"), py::arg("checkpoint_interval"));
+    m.def("generate_rays",
+          [](py::object viewport, float fovY, int W, int H, py::object ray_start, py::object ray_dir) {
+              dvr_tensor v = to_tensor(viewport), s = to_tensor(ray_start), d = to_tensor(ray_dir);
+              check(dvr_generate_rays(&v, fovY, W, H, &s, &d, current_stream(v.device)));
+          });
+    m.def("generate_rays_adjoint",
+          [](py::object viewport, float fovY, int W, int H, py::object adj_start, py::object adj_dir, py::object out) {
+              dvr_tensor v = to_tensor(viewport), s = to_tensor(adj_start), d = to_tensor(adj_dir), o = to_tensor(out);
+              check(dvr_generate_rays_adjoint(&v, fovY, W, H, &s, &d, &o, current_stream(v.device)));
+          });
+}

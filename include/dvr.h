@@ -1,0 +1,206 @@
+/*
+ * dvr.h -- C-ABI of the MI355X differentiable volume renderer (libdvr_hip.so).
+ *
+ * Plain pointers, sizes and strides only; no framework types cross this
+ * boundary.  Every entry point returns a dvr_status; on failure
+ * dvr_last_error() returns a thread-local, human-readable message (the
+ * reference raises c10::Error with the same text through pybind11).
+ *
+ * Semantics follow DiffDVR's renderer (paths relative to that repository):
+ *   - tensors may have arbitrary element strides; a batch dimension of size 1
+ *     broadcasts over the render batch (renderer/renderer_tensor.cuh:122-269,
+ *     renderer/pytorch_utils.h:12-20);
+ *   - the caller allocates every output; adjoint outputs are accumulated into
+ *     and must be zero-initialised by the caller
+ *     (renderer/renderer_settings.cuh:227-228);
+ *   - launches are asynchronous on the given HIP stream (NULL = default stream).
+ *
+ * All floating-point tensors are float32 (the reference's real_t with
+ * USE_DOUBLE_PRECISION=0); terminationIndex is int32.
+ */
+#ifndef DVR_H_
+#define DVR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t dvr_status;
+#define DVR_OK 0
+#define DVR_ERR_INVALID_ARGUMENT 1 /* validation failure (reference: TORCH_CHECK) */
+#define DVR_ERR_NO_GRADIENT 2      /* no adjoint output enabled (renderer.cpp:712-717) */
+#define DVR_ERR_UNSUPPORTED 3      /* combination not available on this device path */
+#define DVR_ERR_HIP 4              /* HIP runtime / launch error */
+
+/* dtypes */
+#define DVR_FLOAT32 0
+#define DVR_INT32 1
+
+/* Enum values are those of renderer/renderer_settings.cuh:81-146 and of the
+ * pyrenderer enums (bindings/bindings.cpp:215-232). */
+#define DVR_FILTER_NEAREST 0
+#define DVR_FILTER_TRILINEAR 1
+#define DVR_CAMERA_RAY_START_DIR 0
+#define DVR_CAMERA_INVERSE_VIEW_MATRIX 1
+#define DVR_CAMERA_REFERENCE_FRAME 2
+#define DVR_TF_IDENTITY 0
+#define DVR_TF_TEXTURE 1
+#define DVR_TF_LINEAR 2
+#define DVR_TF_GAUSSIAN 3
+#define DVR_BLEND_BEER_LAMBERT 0
+#define DVR_BLEND_ALPHA 1
+
+/* A strided view of device (or, where stated, host) memory. */
+typedef struct dvr_tensor {
+    void* data;         /* NULL with ndim == 0 = not given (an empty tensor has ndim >= 1) */
+    int64_t sizes[6];
+    int64_t strides[6]; /* in elements */
+    int32_t ndim;
+    int32_t dtype;      /* DVR_FLOAT32 / DVR_INT32 */
+    int32_t device;     /* HIP device ordinal, -1 = host memory */
+} dvr_tensor;
+
+/* Mirrors renderer::RendererInputsHost (renderer/renderer.h:20-64).
+ * box_min, box_size, step_size and an InverseViewMatrix camera can be given
+ * either as a tensor or as a value (the std::variant alternatives of the
+ * reference): the *_value field is used when the tensor's data is NULL. */
+typedef struct dvr_inputs {
+    int32_t screen_width, screen_height;
+    int32_t volume_filter_mode;
+    dvr_tensor volume;                       /* (B) x X x Y x Z */
+    dvr_tensor box_min;  float box_min_value[3];   /* (B) x 3 */
+    dvr_tensor box_size; float box_size_value[3];  /* (B) x 3 */
+    int32_t camera_mode;
+    dvr_tensor camera_ray_start;             /* (B) x H x W x 3   (RayStartDir) */
+    dvr_tensor camera_ray_dir;               /* (B) x H x W x 3   (RayStartDir) */
+    dvr_tensor camera_matrix;                /* (B) x 4 x 4 (InverseViewMatrix) or (B) x 3 x 3 (ReferenceFrame) */
+    float camera_matrix_value[16];           /* row-major 4x4, InverseViewMatrix given as a value (glm::mat4) */
+    float camera_fov_y_radians;              /* ReferenceFrame only */
+    dvr_tensor step_size; float step_size_value;   /* (B) x (H) x (W) */
+    int32_t tf_mode;
+    dvr_tensor tf;                           /* (B) x R x C */
+    int32_t blend_mode;
+    float blending_early_out;                /* default 1 - 1e-5 */
+} dvr_inputs;
+
+/* Mirrors renderer::RendererOutputsHost (renderer/renderer.h:67-71). */
+typedef struct dvr_outputs {
+    dvr_tensor color;             /* B x H x W x 4 */
+    dvr_tensor termination_index; /* B x H x W, int32 */
+} dvr_outputs;
+
+/* Mirrors renderer::AdjointOutputsHost (renderer/renderer.h:95-111). */
+typedef struct dvr_adjoint_outputs {
+    int32_t has_volume_derivatives;   dvr_tensor adj_volume;          /* (B) x X x Y x Z */
+    int32_t has_stepsize_derivatives; dvr_tensor adj_stepsize;        /* (B) x (H) x (W) */
+    int32_t has_camera_derivatives;   dvr_tensor adj_camera_ray_start;/* (B) x H x W x 3 */
+                                      dvr_tensor adj_camera_ray_dir;  /* (B) x H x W x 3 */
+    int32_t has_tf_derivatives;       dvr_tensor adj_tf;              /* (B) x R x C */
+    int32_t tf_delayed_accumulation;
+} dvr_adjoint_outputs;
+
+/* Kernel family selection. */
+#define DVR_PATH_AUTO 0        /* tiled path where available, else plain */
+#define DVR_PATH_PLAIN 1       /* reference structure: one thread per pixel, row order, grid-stride */
+#define DVR_PATH_TILED 2       /* 8x8 ray tiles per wave64, XCD-aware tile order */
+
+#define DVR_ADJOINT_AUTO 0       /* checkpointed */
+#define DVR_ADJOINT_INVERSION 1  /* reference: reconstruct by inverting the blend step */
+#define DVR_ADJOINT_CHECKPOINT 2 /* store alpha every K samples, recompute segments */
+
+typedef struct dvr_forward_options {
+    int32_t path;                /* DVR_PATH_* */
+    int32_t checkpoint_interval; /* K; 0 = no checkpoints written */
+    dvr_tensor checkpoints;      /* float32 buffer of >= dvr_checkpoint_buffer_elements() */
+} dvr_forward_options;
+
+typedef struct dvr_adjoint_options {
+    int32_t path;                /* DVR_PATH_* */
+    int32_t method;              /* DVR_ADJOINT_* */
+    int32_t checkpoint_interval; /* K in {8,16,32}; 0 = autotuned/default */
+    dvr_tensor checkpoints;      /* optional: written by dvr_render_forward_ex with the same K */
+} dvr_adjoint_options;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* Thread-local message of the last failed call ("" if none). */
+const char* dvr_last_error(void);
+/* 0: this library computes in float32 (bindings/bindings.cpp:143-146). */
+int32_t dvr_use_double_precision(void);
+/* hipDeviceSynchronize (bindings/bindings.cpp:306). */
+dvr_status dvr_sync(void);
+/* Version string of the library build. */
+const char* dvr_version(void);
+
+/* ---- rendering (replaces renderer::Renderer, renderer/renderer.h:159-268) -- */
+
+/* Renderer::renderForward (renderer/renderer.h:159-161; renderer/renderer.cpp:393-417). */
+dvr_status dvr_render_forward(const dvr_inputs* inputs, dvr_outputs* outputs, void* stream);
+
+/* renderForward with a kernel-family choice and optional alpha checkpoints
+ * for a later checkpointed adjoint.  No reference counterpart. */
+dvr_status dvr_render_forward_ex(const dvr_inputs* inputs, dvr_outputs* outputs,
+                                 const dvr_forward_options* options, void* stream);
+
+/* Renderer::renderAdjoint (renderer/renderer.h:264-268; renderer/renderer.cpp:660-742). */
+dvr_status dvr_render_adjoint(const dvr_inputs* inputs, const dvr_outputs* outputs_from_forward,
+                              const dvr_tensor* adj_color, dvr_adjoint_outputs* adj_outputs, void* stream);
+
+/* renderAdjoint with method / path / checkpoint choices. */
+dvr_status dvr_render_adjoint_ex(const dvr_inputs* inputs, const dvr_outputs* outputs_from_forward,
+                                 const dvr_tensor* adj_color, dvr_adjoint_outputs* adj_outputs,
+                                 const dvr_adjoint_options* options, void* stream);
+
+/* Per-pixel image loss whose gradient dvr_render_fused forms in registers. */
+#define DVR_LOSS_L1 1 /* mean |color - target|   (torch.nn.L1Loss)  */
+#define DVR_LOSS_L2 2 /* mean (color - target)^2 (torch.nn.MSELoss) */
+typedef struct dvr_loss {
+    int32_t kind;       /* DVR_LOSS_L1 / DVR_LOSS_L2 */
+    dvr_tensor target;  /* B x H x W x 4 */
+    float scale;        /* factor of the summed loss; 0 = 1/(B*H*W*4), i.e. the mean */
+    dvr_tensor loss;    /* optional 1-element float32: the loss value is ADDED to it */
+} dvr_loss;
+
+/* Forward march, image loss against `target` and checkpointed adjoint in ONE
+ * launch per batch: colour and termination index are written to `outputs`,
+ * the loss gradient never leaves registers, and the adjoint outputs are
+ * accumulated as by dvr_render_adjoint.  checkpoint_interval K in {0,8,16,32}
+ * (0 = default).  No reference counterpart: it fuses Renderer::renderForward,
+ * the caller's torch loss (pytests/tests/volume/compare_reconstruction.py:359-366)
+ * and Renderer::renderAdjoint. */
+dvr_status dvr_render_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss,
+                            dvr_adjoint_outputs* adj_outputs, int32_t checkpoint_interval, void* stream);
+
+/* Number of float32 elements a checkpoint buffer needs for these inputs
+ * and interval K (0 if K == 0). */
+int64_t dvr_checkpoint_buffer_elements(const dvr_inputs* inputs, int32_t checkpoint_interval);
+
+/* ---- camera (replaces renderer/camera_impl.cu:68-93 and :302-357) ------- */
+
+/* Camera::generateRays forward: viewport B x 3 x 3 (eye, right, up) -> per-pixel
+ * ray_start, ray_dir (B x H x W x 3). */
+dvr_status dvr_generate_rays(const dvr_tensor* viewport, float fov_y_radians, int32_t width, int32_t height,
+                             dvr_tensor* ray_start, dvr_tensor* ray_dir, void* stream);
+
+/* GenerateRayFunction::backward: adj_viewport (B x 3 x 3) is OVERWRITTEN with
+ * the per-camera sum of the per-pixel adjoints. */
+dvr_status dvr_generate_rays_adjoint(const dvr_tensor* viewport, float fov_y_radians, int32_t width,
+                                     int32_t height, const dvr_tensor* adj_ray_start,
+                                     const dvr_tensor* adj_ray_dir, dvr_tensor* adj_viewport, void* stream);
+
+/* ---- GPU timer (bindings/bindings.cpp:99-129) ---------------------------- */
+typedef struct dvr_timer dvr_timer;
+dvr_status dvr_timer_create(dvr_timer** timer);
+dvr_status dvr_timer_destroy(dvr_timer* timer);
+dvr_status dvr_timer_start(dvr_timer* timer, void* stream);
+dvr_status dvr_timer_stop(dvr_timer* timer, void* stream);
+dvr_status dvr_timer_elapsed_ms(dvr_timer* timer, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DVR_H_ */

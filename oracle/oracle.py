@@ -1,0 +1,271 @@
+"""ctypes front end of the CPU oracle (liboracle_dvr.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, always as the checker or the timed CPU
+baseline, never as part of the product.
+
+A *scene* is a dict of numpy arrays and modes (all batched arrays carry their
+own leading batch size, 1 = broadcast):
+    W, H, filter, camera, tf_mode, blend, early_out, fov_y
+    volume (vb,X,Y,Z), box_min (nb,3), box_size (nb,3), step (sb,sh,sw), tf (tb,R,C)
+    camera 0: ray_start, ray_dir (rb,H,W,3); 1: cam_matrix (cb,4,4); 2: cam_matrix (cb,3,3)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_dvr.so")
+
+INVERSION = 0
+EXACT_REPLAY = 1
+
+
+class _Inputs(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_int), ("W", ctypes.c_int), ("H", ctypes.c_int),
+        ("filter", ctypes.c_int), ("camera", ctypes.c_int), ("tf_mode", ctypes.c_int), ("blend", ctypes.c_int),
+        ("volume", ctypes.c_void_p), ("vol_b", ctypes.c_int), ("X", ctypes.c_int), ("Y", ctypes.c_int),
+        ("Z", ctypes.c_int),
+        ("box_min", ctypes.c_void_p), ("box_min_b", ctypes.c_int),
+        ("box_size", ctypes.c_void_p), ("box_size_b", ctypes.c_int),
+        ("ray_start", ctypes.c_void_p), ("ray_dir", ctypes.c_void_p), ("ray_b", ctypes.c_int),
+        ("cam_matrix", ctypes.c_void_p), ("cam_b", ctypes.c_int),
+        ("fov_y", ctypes.c_double),
+        ("step", ctypes.c_void_p), ("step_b", ctypes.c_int), ("step_h", ctypes.c_int), ("step_w", ctypes.c_int),
+        ("tf", ctypes.c_void_p), ("tf_b", ctypes.c_int), ("R", ctypes.c_int), ("C", ctypes.c_int),
+        ("early_out", ctypes.c_double),
+    ]
+
+
+class _AdjOut(ctypes.Structure):
+    _fields_ = [
+        ("adj_volume", ctypes.c_void_p), ("adj_volume_b", ctypes.c_int),
+        ("adj_tf", ctypes.c_void_p), ("adj_tf_b", ctypes.c_int),
+        ("adj_ray_start", ctypes.c_void_p), ("adj_ray_dir", ctypes.c_void_p), ("adj_cam_b", ctypes.c_int),
+        ("adj_step", ctypes.c_void_p), ("adj_step_b", ctypes.c_int), ("adj_step_h", ctypes.c_int),
+        ("adj_step_w", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        for suf in ("f64", "f32"):
+            getattr(_lib, f"oracle_tf_adjoint_{suf}").restype = ctypes.c_double if suf == "f64" else ctypes.c_float
+            getattr(_lib, f"oracle_fetch_{suf}").restype = ctypes.c_double if suf == "f64" else ctypes.c_float
+        _lib.oracle_get_threads.restype = ctypes.c_int
+    return _lib
+
+
+def set_threads(n: int):
+    lib().oracle_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().oracle_get_threads()
+
+
+def _suffix(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+def _c(dtype):
+    return ctypes.c_double if np.dtype(dtype) == np.float64 else ctypes.c_float
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def batch_size(scene) -> int:
+    B = 1
+    keys = ["volume", "box_min", "box_size", "step", "tf"]
+    keys += ["ray_start", "ray_dir"] if scene["camera"] == 0 else ["cam_matrix"]
+    for k in keys:
+        b = scene[k].shape[0]
+        if b != 1:
+            if B not in (1, b):
+                raise ValueError(f"batch mismatch at {k}")
+            B = b
+    return B
+
+
+class _Prepared:
+    """Contiguous copies of the scene arrays in one dtype, kept alive with the struct."""
+
+    def __init__(self, scene, dtype):
+        self.dtype = np.dtype(dtype)
+        conv = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, dtype=self.dtype))
+        s = {k: conv(scene.get(k)) for k in ("volume", "box_min", "box_size", "ray_start", "ray_dir", "cam_matrix",
+                                             "step", "tf")}
+        self.arrays = s
+        self.B = batch_size(scene)
+        W, H = int(scene["W"]), int(scene["H"])
+        st = s["step"]
+        st3 = st.reshape(st.shape[0], st.shape[1], st.shape[2])
+        self.s = _Inputs(
+            B=self.B, W=W, H=H, filter=int(scene["filter"]), camera=int(scene["camera"]),
+            tf_mode=int(scene["tf_mode"]), blend=int(scene["blend"]),
+            volume=_ptr(s["volume"]), vol_b=s["volume"].shape[0], X=s["volume"].shape[1], Y=s["volume"].shape[2],
+            Z=s["volume"].shape[3],
+            box_min=_ptr(s["box_min"]), box_min_b=s["box_min"].shape[0],
+            box_size=_ptr(s["box_size"]), box_size_b=s["box_size"].shape[0],
+            ray_start=_ptr(s["ray_start"]), ray_dir=_ptr(s["ray_dir"]),
+            ray_b=s["ray_start"].shape[0] if s["ray_start"] is not None else 1,
+            cam_matrix=_ptr(s["cam_matrix"]), cam_b=s["cam_matrix"].shape[0] if s["cam_matrix"] is not None else 1,
+            fov_y=float(scene.get("fov_y", 0.0)),
+            step=_ptr(st3), step_b=st3.shape[0], step_h=st3.shape[1], step_w=st3.shape[2],
+            tf=_ptr(s["tf"]), tf_b=s["tf"].shape[0], R=s["tf"].shape[1], C=s["tf"].shape[2],
+            early_out=float(scene.get("early_out", 1.0 - 1e-5)),
+        )
+        self.W, self.H = W, H
+
+
+def forward(scene, dtype=np.float64, frozen_term=None, frozen_t0=None, return_t0=False):
+    """Oracle forward render -> color (B,H,W,4), term (B,H,W) int32[, t0 (B,H,W)]."""
+    p = _Prepared(scene, dtype)
+    B, H, W = p.B, p.H, p.W
+    color = np.zeros((B, H, W, 4), dtype=p.dtype)
+    term = np.zeros((B, H, W), dtype=np.int32)
+    ft = None if frozen_term is None else np.ascontiguousarray(frozen_term, dtype=np.int32)
+    f0 = None if frozen_t0 is None else np.ascontiguousarray(frozen_t0, dtype=p.dtype)
+    t0 = np.zeros((B, H, W), dtype=p.dtype) if return_t0 else None
+    getattr(lib(), f"oracle_render_forward_{_suffix(dtype)}")(
+        ctypes.byref(p.s), _ptr(color), _ptr(term), _ptr(ft), _ptr(f0), _ptr(t0))
+    return (color, term, t0) if return_t0 else (color, term)
+
+
+def adjoint(scene, color, term, adj_color, grads=("tf",), dtype=np.float64, method=INVERSION, frozen_t0=None,
+            adj_volume_batch=None, adj_tf_batch=None, adj_cam_batch=None, adj_step_shape=None):
+    """Oracle adjoint -> dict with the requested gradients (accumulated from zero)."""
+    p = _Prepared(scene, dtype)
+    B, H, W = p.B, p.H, p.W
+    vol = p.arrays["volume"]
+    tf = p.arrays["tf"]
+    out = {}
+    s = _AdjOut()
+    if "volume" in grads:
+        vb = adj_volume_batch if adj_volume_batch is not None else vol.shape[0]
+        out["volume"] = np.zeros((vb,) + vol.shape[1:], dtype=p.dtype)
+        s.adj_volume, s.adj_volume_b = _ptr(out["volume"]), vb
+    if "tf" in grads:
+        tb = adj_tf_batch if adj_tf_batch is not None else tf.shape[0]
+        out["tf"] = np.zeros((tb,) + tf.shape[1:], dtype=p.dtype)
+        s.adj_tf, s.adj_tf_b = _ptr(out["tf"]), tb
+    if "camera" in grads:
+        cb = adj_cam_batch if adj_cam_batch is not None else B
+        out["ray_start"] = np.zeros((cb, H, W, 3), dtype=p.dtype)
+        out["ray_dir"] = np.zeros((cb, H, W, 3), dtype=p.dtype)
+        s.adj_ray_start, s.adj_ray_dir, s.adj_cam_b = _ptr(out["ray_start"]), _ptr(out["ray_dir"]), cb
+    if "step" in grads:
+        shp = adj_step_shape if adj_step_shape is not None else (B, H, W)
+        out["step"] = np.zeros(shp, dtype=p.dtype)
+        s.adj_step, s.adj_step_b, s.adj_step_h, s.adj_step_w = _ptr(out["step"]), shp[0], shp[1], shp[2]
+    c = np.ascontiguousarray(color, dtype=p.dtype)
+    t = np.ascontiguousarray(term, dtype=np.int32)
+    a = np.ascontiguousarray(adj_color, dtype=p.dtype)
+    f0 = None if frozen_t0 is None else np.ascontiguousarray(frozen_t0, dtype=p.dtype)
+    getattr(lib(), f"oracle_render_adjoint_{_suffix(dtype)}")(
+        ctypes.byref(p.s), _ptr(c), _ptr(t), _ptr(a), ctypes.byref(s), int(method), _ptr(f0))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# components
+# ---------------------------------------------------------------------------
+def _arr(x, dtype):
+    return np.ascontiguousarray(np.asarray(x, dtype=dtype))
+
+
+def blend(mode, acc, cur, step, dtype=np.float64):
+    out = np.zeros(4, dtype=dtype)
+    getattr(lib(), f"oracle_blend_{_suffix(dtype)}")(int(mode), _ptr(_arr(acc, dtype)), _ptr(_arr(cur, dtype)),
+                                                      _c(dtype)(step), _ptr(out))
+    return out
+
+
+def blend_adjoint(mode, acc_out, cur, step, adj_out, dtype=np.float64):
+    """Inversion adjoint of one blend step -> (acc_in, adj_acc_in, adj_cur, adj_step)."""
+    acc_in, adj_acc, adj_cur = (np.zeros(4, dtype=dtype) for _ in range(3))
+    adj_step = np.zeros(1, dtype=dtype)
+    getattr(lib(), f"oracle_blend_adjoint_{_suffix(dtype)}")(
+        int(mode), _ptr(_arr(acc_out, dtype)), _ptr(_arr(cur, dtype)), _c(dtype)(step), _ptr(_arr(adj_out, dtype)),
+        _ptr(acc_in), _ptr(adj_acc), _ptr(adj_cur), _ptr(adj_step))
+    return acc_in, adj_acc, adj_cur, float(adj_step[0])
+
+
+def tf_eval(mode, tf, density, dtype=np.float64):
+    tf = _arr(tf, dtype)
+    out = np.zeros(4, dtype=dtype)
+    getattr(lib(), f"oracle_tf_eval_{_suffix(dtype)}")(int(mode), _ptr(tf), tf.shape[0], tf.shape[1],
+                                                        _c(dtype)(density), _ptr(out))
+    return out
+
+
+def tf_adjoint(mode, tf, density, adj, want_tf=True, dtype=np.float64):
+    tf = _arr(tf, dtype)
+    g = np.zeros_like(tf) if want_tf else None
+    d = getattr(lib(), f"oracle_tf_adjoint_{_suffix(dtype)}")(int(mode), _ptr(tf), tf.shape[0], tf.shape[1],
+                                                               _c(dtype)(density), _ptr(_arr(adj, dtype)), _ptr(g))
+    return float(d), g
+
+
+def fetch(filter_mode, vol, p, dtype=np.float64):
+    vol = _arr(vol, dtype)
+    return float(getattr(lib(), f"oracle_fetch_{_suffix(dtype)}")(int(filter_mode), _ptr(vol), *vol.shape,
+                                                                   _ptr(_arr(p, dtype))))
+
+
+def trilinear_adjoint(vol, p, adj_d, want_volume=True, dtype=np.float64):
+    vol = _arr(vol, dtype)
+    g = np.zeros_like(vol) if want_volume else None
+    ap = np.zeros(3, dtype=dtype)
+    getattr(lib(), f"oracle_trilinear_adjoint_{_suffix(dtype)}")(_ptr(vol), *vol.shape, _ptr(_arr(p, dtype)),
+                                                                 _c(dtype)(adj_d), _ptr(g), _ptr(ap))
+    return ap, g
+
+
+def adj_cross(a, b, adj_c, dtype=np.float64):
+    aa, ab = np.zeros(3, dtype=dtype), np.zeros(3, dtype=dtype)
+    getattr(lib(), f"oracle_adj_cross_{_suffix(dtype)}")(_ptr(_arr(a, dtype)), _ptr(_arr(b, dtype)),
+                                                          _ptr(_arr(adj_c, dtype)), _ptr(aa), _ptr(ab))
+    return aa, ab
+
+
+def adj_normalize(a, adj_out, dtype=np.float64):
+    r = np.zeros(3, dtype=dtype)
+    getattr(lib(), f"oracle_adj_normalize_{_suffix(dtype)}")(_ptr(_arr(a, dtype)), _ptr(_arr(adj_out, dtype)),
+                                                              _ptr(r))
+    return r
+
+
+def generate_rays(viewport, H, W, fov_y, dtype=np.float64):
+    vp = _arr(viewport, dtype)
+    B = vp.shape[0]
+    rs, rd = np.zeros((B, H, W, 3), dtype=dtype), np.zeros((B, H, W, 3), dtype=dtype)
+    getattr(lib(), f"oracle_generate_rays_{_suffix(dtype)}")(_ptr(vp), B, H, W, _c(dtype)(fov_y), _ptr(rs), _ptr(rd))
+    return rs, rd
+
+
+def generate_rays_adjoint(viewport, H, W, fov_y, adj_start, adj_dir, dtype=np.float64):
+    vp = _arr(viewport, dtype)
+    out = np.zeros_like(vp)
+    getattr(lib(), f"oracle_generate_rays_adjoint_{_suffix(dtype)}")(
+        _ptr(vp), vp.shape[0], H, W, _c(dtype)(fov_y), _ptr(_arr(adj_start, dtype)), _ptr(_arr(adj_dir, dtype)),
+        _ptr(out))
+    return out

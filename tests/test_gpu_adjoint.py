@@ -1,0 +1,222 @@
+"""GPU adjoint parity against the FP64 oracle (renderer_kernels.cuh:315-497).
+
+The oracle replays each ray with the GPU's sample count and keeps every
+pre-sample accumulator (exact replay, no inversion); the error of a gradient
+is the normwise relative error ||g_gpu - g_oracle|| / ||g_oracle||, bound 1e-4
+on smooth volumes.  Paths covered: the plain reference-structured inversion
+adjoint, the tiled checkpointed adjoint (K = 8/16/32, with and without
+checkpoints handed over from the forward), the fused forward+loss+adjoint
+kernel, broadcast gradient tensors, and camera ray generation.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import oracle, scenes, scene_from_workload, camera_scene
+from gpu_helpers import render, gpu_inputs, t32, dev, norm_err, rel_err, batch
+from diffdvr_amd import pyrenderer as pr
+
+pytestmark = pytest.mark.gpu
+
+
+def make_adj(s, grads, shapes=None):
+    shapes = shapes or {}
+    B, H, W = batch(s), int(s["H"]), int(s["W"])
+    a = pr.AdjointOutputs()
+    out = {}
+    if "tf" in grads:
+        a.has_tf_derivatives = True
+        a.tf_delayed_accumulation = True
+        out["tf"] = a.adj_tf = torch.zeros(shapes.get("tf", s["tf"].shape), device=dev())
+    if "volume" in grads:
+        a.has_volume_derivatives = True
+        out["volume"] = a.adj_volume = torch.zeros(shapes.get("volume", s["volume"].shape), device=dev())
+    if "camera" in grads:
+        a.has_camera_derivatives = True
+        cb = shapes.get("camera", B)
+        out["ray_start"] = a.adj_camera_ray_start = torch.zeros(cb, H, W, 3, device=dev())
+        out["ray_dir"] = a.adj_camera_ray_dir = torch.zeros(cb, H, W, 3, device=dev())
+    if "step" in grads:
+        a.has_stepsize_derivatives = True
+        out["step"] = a.adj_stepsize = torch.zeros(shapes.get("step", (B, H, W)), device=dev())
+    return a, out
+
+
+def oracle_grads(s, t_gpu, adj_color, grads, **kw):
+    c_ora, _ = oracle.forward(s, frozen_term=t_gpu)
+    return oracle.adjoint(s, c_ora, t_gpu, adj_color, grads=grads, method=oracle.EXACT_REPLAY, **kw)
+
+
+def compare(got, want, tol, label=""):
+    errs = {}
+    for k, v in want.items():
+        g = got[k].double().cpu().numpy()
+        if np.linalg.norm(v) == 0:
+            assert np.abs(g).max() < 1e-6, (label, k)
+            continue
+        errs[k] = norm_err(g, v)
+        assert errs[k] < tol, (label, k, errs[k])
+    return errs
+
+
+ADJ_PATHS = [("inversion", "plain", 0), ("checkpoint", "tiled", 8), ("checkpoint", "tiled", 16),
+             ("checkpoint", "tiled", 32)]
+
+
+@pytest.mark.parametrize("filter", [0, 1])
+@pytest.mark.parametrize("tf_mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("blend", [0, 1])
+def test_adjoint_matrix(filter, tf_mode, blend):
+    rng = np.random.default_rng(2000 + 100 * filter + 10 * tf_mode + blend)
+    s = camera_scene(rng, 19, 17, B=2, filter=filter, tf_mode=tf_mode, blend=blend, camera=0, smooth=True,
+                     X=16, Y=14, Z=18, step=0.025)
+    if blend == 1:
+        s["tf"][..., 0 if tf_mode == 0 else 3] *= 0.5
+    grads = ("tf", "volume", "camera", "step") if filter == 1 else ("tf", "volume")
+    inp, color, term = render(s)
+    adj_color = rng.standard_normal(color.shape)
+    t_gpu = term.cpu().numpy()
+    want = oracle_grads(s, t_gpu, adj_color, grads)
+    for method, path, K in ADJ_PATHS:
+        a, got = make_adj(s, grads)
+        pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), t32(adj_color), a, method=method,
+                                   path=path, checkpoint_interval=K)
+        torch.cuda.synchronize()
+        compare(got, want, 1e-4, (method, path, K))
+
+
+@pytest.mark.parametrize("K", [8, 16, 32])
+def test_c2_fused_l1_tf_gradient(K):
+    wl = scenes.workload_c2(views=2, size=64)
+    s = scene_from_workload(wl)
+    s_t = scene_from_workload(wl, tf=wl.target_tf)
+    B, H, W = 2, 64, 64
+    _, target, _ = render(s_t)
+    inp = gpu_inputs(s)
+    color = torch.empty(B, H, W, 4, device=dev())
+    term = torch.empty(B, H, W, dtype=torch.int32, device=dev())
+    a, got = make_adj(s, ("tf",))
+    loss = torch.zeros(1, device=dev())
+    pr.Renderer.render_fused(inp, pr.RendererOutputs(color, term), "l1", target, a, loss_out=loss,
+                             checkpoint_interval=K)
+    torch.cuda.synchronize()
+    t_gpu = term.cpu().numpy()
+    c_ora, t_free = oracle.forward(s)
+    assert np.mean(t_free == t_gpu) > 0.999
+    c_ora, _ = oracle.forward(s, frozen_term=t_gpu)
+    assert rel_err(color.double().cpu().numpy(), c_ora) < 1e-5
+    tg = target.double().cpu().numpy()
+    adj_color = np.sign(color.double().cpu().numpy() - tg) / c_ora.size
+    want = oracle.adjoint(s, c_ora, t_gpu, adj_color, grads=("tf",), method=oracle.EXACT_REPLAY)
+    err = compare(got, want, 1e-4, "fused")
+    assert float(loss) == pytest.approx(np.abs(c_ora - tg).mean(), rel=1e-4)
+    print(f"C2 small fused K={K}: tf grad rel err {err['tf']:.2e}")
+
+
+def test_fused_l2_volume_gradient():
+    wl = scenes.workload_c3(views=3, size=40, res=32)
+    s = scene_from_workload(wl)
+    s_t = scene_from_workload(wl, volume=wl.target_volume)
+    _, target, _ = render(s_t)
+    inp = gpu_inputs(s)
+    B, H, W = 3, 40, 40
+    color = torch.empty(B, H, W, 4, device=dev())
+    term = torch.empty(B, H, W, dtype=torch.int32, device=dev())
+    a, got = make_adj(s, ("volume",))
+    pr.Renderer.render_fused(inp, pr.RendererOutputs(color, term), "l2", target, a)
+    torch.cuda.synchronize()
+    t_gpu = term.cpu().numpy()
+    c_ora, _ = oracle.forward(s, frozen_term=t_gpu)
+    adj_color = 2.0 * (color.double().cpu().numpy() - target.double().cpu().numpy()) / c_ora.size
+    want = oracle.adjoint(s, c_ora, t_gpu, adj_color, grads=("volume",), method=oracle.EXACT_REPLAY)
+    compare(got, want, 1e-4, "fused-l2")
+
+
+def test_forward_checkpoints_handoff():
+    rng = np.random.default_rng(31)
+    s = camera_scene(rng, 33, 29, B=2, tf_mode=2, camera=2, smooth=True, X=20, Y=20, Z=20, step=0.01)
+    inp = gpu_inputs(s)
+    B, H, W = 2, 29, 33
+    K = 16
+    n = pr.Renderer.checkpoint_buffer_elements(inp, K)
+    ck = torch.full((n,), float("nan"), device=dev())
+    color = torch.empty(B, H, W, 4, device=dev())
+    term = torch.empty(B, H, W, dtype=torch.int32, device=dev())
+    pr.Renderer.render_forward(inp, pr.RendererOutputs(color, term), path="tiled", checkpoint_interval=K,
+                               checkpoints=ck)
+    adj_color = t32(rng.standard_normal((B, H, W, 4)))
+    a1, g1 = make_adj(s, ("tf", "volume"))
+    pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), adj_color, a1, checkpoint_interval=K,
+                               checkpoints=ck)
+    a2, g2 = make_adj(s, ("tf", "volume"))
+    pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), adj_color, a2, checkpoint_interval=K)
+    torch.cuda.synchronize()
+    for k in g1:
+        assert norm_err(g1[k].cpu().numpy(), g2[k].cpu().numpy()) < 1e-6
+    want = oracle_grads(s, term.cpu().numpy(), adj_color.double().cpu().numpy(), ("tf", "volume"))
+    compare(g1, want, 1e-4)
+
+
+def test_broadcast_gradient_outputs():
+    rng = np.random.default_rng(41)
+    s = camera_scene(rng, 17, 15, B=1, tf_mode=1, camera=0, smooth=True, X=16, Y=16, Z=16)
+    # 3 views sharing ray tensors of batch 1, per-view TF
+    s["tf"] = np.concatenate([s["tf"], s["tf"] * 0.8, s["tf"] * 1.2], axis=0)
+    inp, color, term = render(s)
+    adj_color = rng.standard_normal(color.shape)
+    t_gpu = term.cpu().numpy()
+    shapes = {"volume": s["volume"].shape, "camera": 1, "step": (1, 1, 1)}
+    want = oracle_grads(s, t_gpu, adj_color, ("tf", "volume", "camera", "step"), adj_cam_batch=1,
+                        adj_step_shape=(1, 1, 1))
+    for method, path, K in ADJ_PATHS[:2]:
+        a, got = make_adj(s, ("tf", "volume", "camera", "step"), shapes)
+        pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), t32(adj_color), a, method=method,
+                                   path=path, checkpoint_interval=K)
+        torch.cuda.synchronize()
+        compare(got, want, 1e-4, method)
+
+
+def test_checkpoint_beats_inversion_near_opacity():
+    rng = np.random.default_rng(51)
+    s = camera_scene(rng, 24, 24, B=1, tf_mode=1, camera=2, smooth=True, X=24, Y=24, Z=24, step=0.01)
+    s["tf"][..., 3] *= 12.0
+    inp, color, term = render(s)
+    assert float(color[..., 3].max()) > 0.999
+    adj_color = rng.standard_normal(color.shape)
+    want = oracle_grads(s, term.cpu().numpy(), adj_color, ("tf", "volume"))
+    errs = {}
+    for method, path, K in (ADJ_PATHS[0], ADJ_PATHS[2]):
+        a, got = make_adj(s, ("tf", "volume"))
+        pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), t32(adj_color), a, method=method,
+                                   path=path, checkpoint_interval=K)
+        torch.cuda.synchronize()
+        errs[method] = {k: norm_err(got[k].double().cpu().numpy(), want[k]) for k in want}
+    print("near-opacity gradient errors:", errs)
+    assert max(errs["checkpoint"].values()) < 1e-4
+
+
+def test_no_gradient_raises():
+    rng = np.random.default_rng(61)
+    s = camera_scene(rng, 8, 8, tf_mode=1, camera=2)
+    inp, color, term = render(s)
+    with pytest.raises(RuntimeError, match="At least one adjoint variable must be activated"):
+        pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), torch.zeros_like(color),
+                                   pr.AdjointOutputs())
+
+
+def test_generate_rays_and_adjoint():
+    rng = np.random.default_rng(71)
+    vp = scenes.sphere_viewports(rng.uniform(-1, 1, 3), rng.uniform(0, 6, 3), 2.5)
+    H, W, fov = 23, 37, math.radians(45)
+    rs, rd = pr.Camera.generate_rays(vp.to(dev()), fov, W, H)
+    ors, ord_ = oracle.generate_rays(vp.double().numpy(), H, W, fov)
+    assert rel_err(rs.cpu().numpy(), ors) < 1e-6
+    assert np.abs(rd.cpu().numpy() - ord_).max() < 1e-6
+    vpd = vp.to(dev()).requires_grad_(True)
+    rs, rd = pr.Camera.generate_rays(vpd, fov, W, H)
+    a_s, a_d = rng.standard_normal(rs.shape), rng.standard_normal(rd.shape)
+    ((rs * t32(a_s)).sum() + (rd * t32(a_d)).sum()).backward()
+    want = oracle.generate_rays_adjoint(vp.double().numpy(), H, W, fov, a_s, a_d)
+    assert norm_err(vpd.grad.double().cpu().numpy(), want) < 1e-5
