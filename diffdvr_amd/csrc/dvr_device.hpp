@@ -304,6 +304,76 @@ __device__ __forceinline__ void blend(f4& acc, const f4& s, float step)
     acc.w = acc.w + tr * a;
 }
 
+// ---------------------------------------------------------------------------
+// Transmittance-tracking form used by the tiled kernels.  Mathematically the
+// same blend as above, evaluated so that FP32 keeps its relative precision:
+//   alpha = 1 - exp(-x) is formed with expm1 for small x (no cancellation),
+//   keep  = 1 - alpha   is exp(-x) itself,
+//   T     = 1 - A       is carried multiplicatively (T *= keep), so factors
+//                       (1 - A) stay accurate as A approaches 1, while A is
+//                       summed (A += T alpha) and stays accurate while small.
+// ---------------------------------------------------------------------------
+struct Acc {
+    float r, g, b, A, T;
+};
+
+template <int BLEND>
+__device__ __forceinline__ void alpha_keep(float absorption, float step, float& alpha, float& keep)
+{
+    if constexpr (BLEND == 0) {
+        const float x = absorption * step;
+        keep = expf(-x);
+        alpha = x < 0.5f ? -expm1f(-x) : 1.0f - keep;
+    } else {
+        alpha = tmin_(1.0f, absorption * step);
+        keep = 1.0f - alpha;
+    }
+}
+
+template <int BLEND>
+__device__ __forceinline__ void blend_t(Acc& acc, const f4& s, float step)
+{
+    float alpha, keep;
+    alpha_keep<BLEND>(s.w, step, alpha, keep);
+    const float w = acc.T * alpha;
+    acc.r += w * s.x;
+    acc.g += w * s.y;
+    acc.b += w * s.z;
+    acc.A += w;
+    acc.T *= keep;
+}
+
+// Blend adjoint given the transmittance before the sample (t_in = 1 - a_in);
+// same derivative as blend_adjoint.  The alpha-mode clamp at 1-1e-5 follows
+// renderer_blending.cuh:136,158-159.
+template <int BLEND>
+__device__ __forceinline__ void blend_adjoint_t(float t_in, const f4& s, float step, f4& adj_acc, f4& adj_s,
+                                                float& adj_step)
+{
+    float alpha, keep;
+    if constexpr (BLEND == 0) {
+        alpha_keep<0>(s.w, step, alpha, keep);
+    } else {
+        alpha = tmin_(1.0f - 1e-5f, s.w * step);
+        keep = 1.0f - alpha;
+    }
+    const float dotc = adj_acc.x * s.x + adj_acc.y * s.y + adj_acc.z * s.z;
+    const float adj_alpha = t_in * (adj_acc.w + dotc);
+    const float k = alpha * t_in;
+    adj_s.x = adj_acc.x * k;
+    adj_s.y = adj_acc.y * k;
+    adj_s.z = adj_acc.z * k;
+    if constexpr (BLEND == 0) {
+        adj_s.w = adj_alpha * step * keep;
+        adj_step = adj_alpha * s.w * keep;
+    } else {
+        const bool lin = s.w * step < 1.0f;
+        adj_step = lin ? adj_alpha * s.w : 0.0f;
+        adj_s.w = lin ? adj_alpha * step : 0.0f;
+    }
+    adj_acc.w = adj_acc.w * keep - alpha * dotc;
+}
+
 // Blend adjoint for one sample given the accumulated alpha before it (a_in).
 // adj_acc.xyz is constant along the ray; adj_acc.w is propagated in place.
 // (renderer_blending.cuh:63-98 / :128-164 with a_in supplied instead of

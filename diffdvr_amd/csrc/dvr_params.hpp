@@ -52,8 +52,8 @@ struct AdjointParams {
     int tfDelayed;
 };
 
-// Alpha checkpoints: ckpt c (0-based) of a ray holds the accumulated alpha
-// after (c+1)*K samples.  Layout [tile][c][thread] so one wave stores 256 B.
+// Checkpoints: ckpt c (0-based) of a ray holds the transmittance 1 - alpha
+// accumulated after (c+1)*K samples.  Layout [tile][c][thread] so one wave stores 256 B.
 struct CkptParams {
     float* buf;   // NULL = none
     int K;        // checkpoint interval

@@ -48,17 +48,20 @@ __device__ __forceinline__ void stage_tf(const RenderParams& P, int b, float* s_
 }
 
 // ---------------------------------------------------------------------------
-// Forward march of one ray.  frozen >= 0 replays exactly that many samples.
-// Writes the accumulated alpha after every K-th sample to ck[c*ckS] for
-// c < cap (ck may be NULL).
+// Forward march of one ray (transmittance-tracking blend, dvr_device.hpp).
+// frozen >= 0 replays exactly that many samples.  Writes the transmittance
+// after every K-th sample to ck[c*ckS] for c < cap (ck may be NULL).
+// Early termination tests the accumulated alpha as the reference does
+// (renderer_kernels.cuh:64), evaluated as 1 - T.
 // ---------------------------------------------------------------------------
 template <int FILTER, int TFM, int BLEND>
 __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const Ray& r, float earlyOut, int frozen,
-                                              f4& acc, int& n, float* ck, int ckS, int K, int cap)
+                                              Acc& acc, int& n, float* ck, int ckS, int K, int cap)
 {
     Cell c;
     cell_invalidate(c);
-    acc = mk4(0.f, 0.f, 0.f, 0.f);
+    acc = Acc{0.f, 0.f, 0.f, 0.f, 1.f};
+    const float tEarly = 1.0f - earlyOut;
     int countdown = K, cidx = 0;
     int i = 0;
     for (; i < kMaxSamples; ++i) {
@@ -67,14 +70,14 @@ __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const R
             if (i >= frozen) break;
         } else {
             if (t > r.t1) break;
-            if (acc.w > earlyOut) break;
+            if (acc.T < tEarly) break;
         }
         const float d = fetch<FILTER>(V, sample_pos(r, t), c);
-        blend<BLEND>(acc, tf_eval<TFM>(T, d), r.step);
+        blend_t<BLEND>(acc, tf_eval<TFM>(T, d), r.step);
         if (ck) {
             if (--countdown == 0) {
                 countdown = K;
-                if (cidx < cap) ck[cidx * ckS] = acc.w;
+                if (cidx < cap) ck[cidx * ckS] = acc.T;
                 ++cidx;
             }
         }
@@ -167,7 +170,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
     wantTf = CT_TF && wantTf;
     wantVol = CT_VOL && wantVol;
     wantPos = CT_POS && wantPos;
-    float dv[K], ain[K];
+    float dv[K], tin[K];
     // reverse-walk cell state for the volume / position adjoint
     int rcx = kInvalidCell, rcy = 0, rcz = 0;
     float rw[8], rd[8];
@@ -199,20 +202,22 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int i0 = seg * K;
         const int cnt = min(K, n - i0);
-        // accumulated alpha before sample i0
-        float A = 0.0f;
+        // transmittance before sample i0
+        float Tr = 1.0f;
         {
             const int c = min(seg, cap);
-            if (c > 0) A = ck[(c - 1) * ckS];
+            if (c > 0) Tr = ck[(c - 1) * ckS];
             Cell cc;
             cell_invalidate(cc);
             for (int i = c * K; i < i0; ++i) {  // only when the ray outgrew the checkpoint capacity
                 const float t = r.t0 + float(i) * r.step;
                 const float d = fetch<FILTER>(V, sample_pos(r, t), cc);
-                A = A + (1.0f - A) * blend_alpha<BLEND>(tf_eval<TFM>(T, d).w, r.step);
+                float alpha, keep;
+                alpha_keep<BLEND>(tf_eval<TFM>(T, d).w, r.step, alpha, keep);
+                Tr *= keep;
             }
         }
-        // recompute the segment forward, keep density and alpha-before per sample
+        // recompute the segment forward, keep density and transmittance-before per sample
         {
             Cell cc;
             cell_invalidate(cc);
@@ -221,8 +226,10 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 if (j < cnt) {
                     const float t = r.t0 + float(i0 + j) * r.step;
                     dv[j] = fetch<FILTER>(V, sample_pos(r, t), cc);
-                    ain[j] = A;
-                    A = A + (1.0f - A) * blend_alpha<BLEND>(tf_eval<TFM>(T, dv[j]).w, r.step);
+                    tin[j] = Tr;
+                    float alpha, keep;
+                    alpha_keep<BLEND>(tf_eval<TFM>(T, dv[j]).w, r.step, alpha, keep);
+                    Tr *= keep;
                 }
             }
         }
@@ -235,7 +242,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 const f4 s = tf_eval<TFM>(T, dv[j]);
                 f4 adj_s;
                 float adj_st;
-                blend_adjoint<BLEND>(ain[j], s, r.step, adj_acc, adj_s, adj_st);
+                blend_adjoint_t<BLEND>(tin[j], s, r.step, adj_acc, adj_s, adj_st);
                 float adj_d;
                 if (wantTf) adj_d = tf_adjoint<TFM, CT_TF>(T, dv[j], adj_s, sink);
                 else {
@@ -313,11 +320,11 @@ __global__ __launch_bounds__(kBlock) void k_forward_tiled(RenderParams P, CkptPa
     const Vol V = make_vol(P, tc.b);
     const Tf T{lds, P.R, P.C, P.C, 1};
     float* ck = CK.buf ? CK.buf + (size_t(tc.tile) * CK.cap) * kBlock + threadIdx.x : nullptr;
-    f4 acc;
+    Acc acc;
     int n;
     march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, CK.K, CK.cap);
     float* o = P.color + tc.b * P.colSb + tc.y * P.colSy + tc.x * P.colSx;
-    o[0] = acc.x; o[P.colSc] = acc.y; o[2 * P.colSc] = acc.z; o[3 * P.colSc] = acc.w;
+    o[0] = acc.r; o[P.colSc] = acc.g; o[2 * P.colSc] = acc.b; o[3 * P.colSc] = acc.A;
     P.term[tc.b * P.termSb + tc.y * P.termSy + tc.x * P.termSx] = n;
 }
 
@@ -352,14 +359,14 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
         int n;
         f4 adj_acc;
         if (fused) {
-            f4 acc;
+            Acc acc;
             march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, K, CK.cap);
             float* o = P.color + b * P.colSb + y * P.colSy + x * P.colSx;
-            o[0] = acc.x; o[P.colSc] = acc.y; o[2 * P.colSc] = acc.z; o[3 * P.colSc] = acc.w;
+            o[0] = acc.r; o[P.colSc] = acc.g; o[2 * P.colSc] = acc.b; o[3 * P.colSc] = acc.A;
             P.term[b * P.termSb + y * P.termSy + x * P.termSx] = n;
             const float* tg = L.target + b * L.tgSb + y * L.tgSy + x * L.tgSx;
-            const float d0 = acc.x - tg[0], d1 = acc.y - tg[L.tgSc], d2 = acc.z - tg[2 * L.tgSc],
-                        d3 = acc.w - tg[3 * L.tgSc];
+            const float d0 = acc.r - tg[0], d1 = acc.g - tg[L.tgSc], d2 = acc.b - tg[2 * L.tgSc],
+                        d3 = acc.A - tg[3 * L.tgSc];
             if (L.kind == DVR_LOSS_L1) {
                 auto sgn = [](float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); };
                 adj_acc = mk4(sgn(d0) * L.scale, sgn(d1) * L.scale, sgn(d2) * L.scale, sgn(d3) * L.scale);
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
             const float* ac = A.acol + b * A.acSb + y * A.acSy + x * A.acSx;
             adj_acc = mk4(ac[0], ac[A.acSc], ac[2 * A.acSc], ac[3 * A.acSc]);
             if (!ckFromFwd) {
-                f4 acc;
+                Acc acc;
                 int n2;
                 march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, n, acc, n2, ck, kBlock, K, CK.cap);
             }
