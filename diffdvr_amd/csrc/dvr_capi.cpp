@@ -201,6 +201,11 @@ Checked check_inputs(const dvr_inputs* in)
         P.cm = (const float*)m.data;
         P.cmSb = bstride(m, 0); P.cmSr = (int)m.strides[1]; P.cmSc = (int)m.strides[2];
         P.fovY = in->camera_fov_y_radians;
+        if (c.H > 0) {
+            const double fovX = double(P.fovY) * double(c.W) / double(c.H);
+            P.tanX = std::tan(fovX / 2.0);
+            P.tanY = std::tan(double(P.fovY) / 2.0);
+        }
     } break;
     default:
         fail(DVR_ERR_INVALID_ARGUMENT, "unknown camera enum value");

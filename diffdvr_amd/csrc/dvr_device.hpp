@@ -6,8 +6,8 @@
 //   interpolation     renderer/renderer_interpolation.cuh:33-389
 //   transfer function renderer/renderer_tf.cuh:59-648
 //   blending          renderer/renderer_blending.cuh:36-165
-// The implementation is this project's own: plain float math, accurate
-// expf/tanf (no fast-math intrinsics), 32-bit offsets.
+// The implementation is this project's own: float march with accurate
+// expf/expm1f (no fast-math intrinsics), double per-ray setup, 32-bit offsets.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "dvr_params.hpp"
@@ -58,7 +58,11 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks)
 }
 
 // ---------------------------------------------------------------------------
-// Per-ray setup (renderer_kernels.cuh:31-48)
+// Per-ray setup (renderer_kernels.cuh:31-48).  Once per ray, the camera ray
+// and the slab test are evaluated in double and rounded to float: the
+// per-sample march then starts from correctly rounded ray parameters (a few
+// ulp of direction error shift distant samples by ~1e-5 voxel, which sharp
+// transfer functions amplify).  The march itself is float.
 // ---------------------------------------------------------------------------
 struct Ray {
     f3 start, dir;
@@ -66,25 +70,40 @@ struct Ray {
     float step, t0, t1;
 };
 
-__device__ __forceinline__ f3 matmul_proj(const float m[16], float x, float y, float z)
+struct d3 {
+    double x, y, z;
+};
+__device__ __forceinline__ d3 dk3(double x, double y, double z) { return d3{x, y, z}; }
+__device__ __forceinline__ d3 dcross(d3 a, d3 b)
 {
-    const float w0 = m[0] * x + m[1] * y + m[2] * z + m[3];
-    const float w1 = m[4] * x + m[5] * y + m[6] * z + m[7];
-    const float w2 = m[8] * x + m[9] * y + m[10] * z + m[11];
-    const float w3 = m[12] * x + m[13] * y + m[14] * z + m[15];
-    return mk3(w0 / w3, w1 / w3, w2 / w3);
+    return dk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ d3 matmul_proj_d(const double m[16], double x, double y, double z)
+{
+    const double w0 = m[0] * x + m[1] * y + m[2] * z + m[3];
+    const double w1 = m[4] * x + m[5] * y + m[6] * z + m[7];
+    const double w2 = m[8] * x + m[9] * y + m[10] * z + m[11];
+    const double w3 = m[12] * x + m[13] * y + m[14] * z + m[15];
+    return dk3(w0 / w3, w1 / w3, w2 / w3);
+}
+
+__device__ __forceinline__ d3 dnormalize(d3 v)
+{
+    const double inv = 1.0 / sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    return dk3(v.x * inv, v.y * inv, v.z * inv);
 }
 
 // Camera from the standard parameter block (renderer_camera.cuh:12-61).
-__device__ __forceinline__ void camera_ray(const RenderParams& P, int x, int y, int b, f3& start, f3& dir)
+__device__ __forceinline__ void camera_ray(const RenderParams& P, int x, int y, int b, d3& start, d3& dir)
 {
     if (P.camMode == 0) {  // RayStartDir
         const float* s = P.rs + b * P.rsSb + y * P.rsSy + x * P.rsSx;
         const float* d = P.rd + b * P.rdSb + y * P.rdSy + x * P.rdSx;
-        start = mk3(s[0], s[P.rsSc], s[2 * P.rsSc]);
-        dir = mk3(d[0], d[P.rdSc], d[2 * P.rdSc]);
+        start = dk3(s[0], s[P.rsSc], s[2 * P.rsSc]);
+        dir = dk3(d[0], d[P.rdSc], d[2 * P.rdSc]);
     } else if (P.camMode == 1) {  // InverseViewMatrix
-        float m[16];
+        double m[16];
         if (P.cm) {
             const float* c = P.cm + b * P.cmSb;
 #pragma unroll
@@ -95,59 +114,68 @@ __device__ __forceinline__ void camera_ray(const RenderParams& P, int x, int y, 
 #pragma unroll
             for (int k = 0; k < 16; ++k) m[k] = P.cmU[k];
         }
-        const float fx = 2.0f * (x + 0.5f) / float(P.W) - 1.0f;
-        const float fy = 2.0f * (y + 0.5f) / float(P.H) - 1.0f;
-        start = matmul_proj(m, fx, fy, 0.0f);
-        const f3 end = matmul_proj(m, fx, fy, 0.9f);
-        dir = normalize3(sub3(end, start));
+        const double fx = 2.0 * (x + 0.5) / double(P.W) - 1.0;
+        const double fy = 2.0 * (y + 0.5) / double(P.H) - 1.0;
+        start = matmul_proj_d(m, fx, fy, 0.0);
+        const d3 end = matmul_proj_d(m, fx, fy, 0.9);
+        dir = dnormalize(dk3(end.x - start.x, end.y - start.y, end.z - start.z));
     } else {  // ReferenceFrame
         const float* c = P.cm + b * P.cmSb;
-        const f3 eye = mk3(c[0], c[P.cmSc], c[2 * P.cmSc]);
-        const f3 right = mk3(c[P.cmSr], c[P.cmSr + P.cmSc], c[P.cmSr + 2 * P.cmSc]);
-        const f3 up = mk3(c[2 * P.cmSr], c[2 * P.cmSr + P.cmSc], c[2 * P.cmSr + 2 * P.cmSc]);
-        const float fovX = P.fovY * float(P.W) / float(P.H);
-        const float tanX = tanf(fovX * 0.5f), tanY = tanf(P.fovY * 0.5f);
-        const f3 front = cross3(up, right);
-        const float fx = 2.0f * (x + 0.5f) / float(P.W) - 1.0f;
-        const float fy = 2.0f * (y + 0.5f) / float(P.H) - 1.0f;
+        const d3 eye = dk3(c[0], c[P.cmSc], c[2 * P.cmSc]);
+        const d3 right = dk3(c[P.cmSr], c[P.cmSr + P.cmSc], c[P.cmSr + 2 * P.cmSc]);
+        const d3 up = dk3(c[2 * P.cmSr], c[2 * P.cmSr + P.cmSc], c[2 * P.cmSr + 2 * P.cmSc]);
+        const d3 front = dcross(up, right);
+        const double fx = (2.0 * (x + 0.5) / double(P.W) - 1.0) * P.tanX;
+        const double fy = (2.0 * (y + 0.5) / double(P.H) - 1.0) * P.tanY;
         start = eye;
-        dir = normalize3(add3(add3(front, scl3(right, fx * tanX)), scl3(up, fy * tanY)));
+        dir = dnormalize(dk3(front.x + fx * right.x + fy * up.x, front.y + fx * right.y + fy * up.y,
+                             front.z + fx * right.z + fy * up.z));
     }
 }
 
-// Slab test with the box given as min/size (renderer_utils.cuh:60-74).
-__device__ __forceinline__ void slab(f3 start, f3 dir, f3 bmin, f3 bsize, float& t0, float& t1)
+// Slab test with the box given as min/size (renderer_utils.cuh:60-74), in double.
+__device__ __forceinline__ void slab(d3 start, d3 dir, d3 bmin, d3 bsize, double& t0, double& t1)
 {
-    const float ix = 1.0f / dir.x, iy = 1.0f / dir.y, iz = 1.0f / dir.z;
-    const float a1 = (bmin.x - start.x) * ix, a2 = (bmin.x + bsize.x - start.x) * ix;
-    const float a3 = (bmin.y - start.y) * iy, a4 = (bmin.y + bsize.y - start.y) * iy;
-    const float a5 = (bmin.z - start.z) * iz, a6 = (bmin.z + bsize.z - start.z) * iz;
-    t0 = tmax_(tmax_(tmin_(a1, a2), tmin_(a3, a4)), tmin_(a5, a6));
-    t1 = tmin_(tmin_(tmax_(a1, a2), tmax_(a3, a4)), tmax_(a5, a6));
+    const double ix = 1.0 / dir.x, iy = 1.0 / dir.y, iz = 1.0 / dir.z;
+    const double a1 = (bmin.x - start.x) * ix, a2 = (bmin.x + bsize.x - start.x) * ix;
+    const double a3 = (bmin.y - start.y) * iy, a4 = (bmin.y + bsize.y - start.y) * iy;
+    const double a5 = (bmin.z - start.z) * iz, a6 = (bmin.z + bsize.z - start.z) * iz;
+    auto mn = [](double a, double b) { return a < b ? a : b; };
+    auto mx = [](double a, double b) { return a > b ? a : b; };
+    t0 = mx(mx(mn(a1, a2), mn(a3, a4)), mn(a5, a6));
+    t1 = mn(mn(mx(a1, a2), mx(a3, a4)), mx(a5, a6));
 }
 
 __device__ __forceinline__ Ray setup_ray(const RenderParams& P, int x, int y, int b)
 {
     Ray r;
-    f3 bsize;
+    d3 bmin, bsize;
     if (P.bmin) {
         const float* p = P.bmin + b * P.bminSb;
-        r.bmin = mk3(p[0], p[P.bminSc], p[2 * P.bminSc]);
+        bmin = dk3(p[0], p[P.bminSc], p[2 * P.bminSc]);
     } else {
-        r.bmin = mk3(P.bminU[0], P.bminU[1], P.bminU[2]);
+        bmin = dk3(P.bminU[0], P.bminU[1], P.bminU[2]);
     }
     if (P.bsize) {
         const float* p = P.bsize + b * P.bsizeSb;
-        bsize = mk3(p[0], p[P.bsizeSc], p[2 * P.bsizeSc]);
+        bsize = dk3(p[0], p[P.bsizeSc], p[2 * P.bsizeSc]);
     } else {
-        bsize = mk3(P.bsizeU[0], P.bsizeU[1], P.bsizeU[2]);
+        bsize = dk3(P.bsizeU[0], P.bsizeU[1], P.bsizeU[2]);
     }
-    r.voxel = div3(bsize, mk3(float(P.X - 1), float(P.Y - 1), float(P.Z - 1)));
+    r.bmin = mk3(float(bmin.x), float(bmin.y), float(bmin.z));
+    r.voxel = div3(mk3(float(bsize.x), float(bsize.y), float(bsize.z)),
+                   mk3(float(P.X - 1), float(P.Y - 1), float(P.Z - 1)));
     r.invVoxel = mk3(1.0f / r.voxel.x, 1.0f / r.voxel.y, 1.0f / r.voxel.z);
-    camera_ray(P, x, y, b, r.start, r.dir);
+    d3 start, dir;
+    camera_ray(P, x, y, b, start, dir);
+    r.start = mk3(float(start.x), float(start.y), float(start.z));
+    r.dir = mk3(float(dir.x), float(dir.y), float(dir.z));
     r.step = P.st ? P.st[b * P.stSb + y * P.stSy + x * P.stSx] : P.stU;
-    slab(r.start, r.dir, r.bmin, bsize, r.t0, r.t1);
-    r.t0 = tmax_(r.t0, 0.0f);
+    double t0, t1;
+    slab(start, dir, bmin, bsize, t0, t1);
+    t0 = t0 > 0.0 ? t0 : 0.0;
+    r.t0 = float(t0);
+    r.t1 = float(t1);
     // Degenerate rays (zero direction, NaN slab bounds) and non-positive or
     // non-finite steps would never leave the march loop: they sample nothing.
     if (!(r.t1 >= r.t0) || !isfinite(r.t0) || !isfinite(r.t1) || !(r.step > 0.0f) || !isfinite(r.step)) {

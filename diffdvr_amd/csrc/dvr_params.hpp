@@ -27,6 +27,7 @@ struct RenderParams {
     const float* rd; int rdSb, rdSy, rdSx, rdSc;
     const float* cm; int cmSb, cmSr, cmSc; float cmU[16];
     float fovY;
+    double tanX, tanY;   // tan(fovX/2), tan(fovY/2) for ReferenceFrame, evaluated on the host in double
     // step size (B) x (H) x (W); pointer NULL -> uniform value
     const float* st; int stSb, stSy, stSx; float stU;
     // transfer function (B) x R x C

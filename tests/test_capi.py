@@ -1,0 +1,139 @@
+"""The C-ABI boundary on the CPU (no GPU calls): libdvr_hip.so loads, exports
+every function include/dvr.h declares, and rejects invalid calls with a
+status code and a message (renderer/renderer.cpp:48-391 validation order),
+never silently (the reference ignores launch errors, renderer_cuda.cpp:109-110).
+Every call here fails validation before any HIP call is made.
+"""
+import ctypes
+import os
+import re
+
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dvr.h")
+LIB = os.path.join(ROOT, "diffdvr_amd", "libdvr_hip.so")
+
+# these tests fabricate device descriptors; never let them near a real device
+pytestmark = pytest.mark.skipif(torch.cuda.is_available(), reason="validation-only tests run where no GPU is visible")
+
+
+class Tensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("sizes", ctypes.c_int64 * 6), ("strides", ctypes.c_int64 * 6),
+                ("ndim", ctypes.c_int32), ("dtype", ctypes.c_int32), ("device", ctypes.c_int32)]
+
+
+class Inputs(ctypes.Structure):
+    _fields_ = [("screen_width", ctypes.c_int32), ("screen_height", ctypes.c_int32),
+                ("volume_filter_mode", ctypes.c_int32), ("volume", Tensor),
+                ("box_min", Tensor), ("box_min_value", ctypes.c_float * 3),
+                ("box_size", Tensor), ("box_size_value", ctypes.c_float * 3),
+                ("camera_mode", ctypes.c_int32), ("camera_ray_start", Tensor), ("camera_ray_dir", Tensor),
+                ("camera_matrix", Tensor), ("camera_matrix_value", ctypes.c_float * 16),
+                ("camera_fov_y_radians", ctypes.c_float), ("step_size", Tensor), ("step_size_value", ctypes.c_float),
+                ("tf_mode", ctypes.c_int32), ("tf", Tensor), ("blend_mode", ctypes.c_int32),
+                ("blending_early_out", ctypes.c_float)]
+
+
+class Outputs(ctypes.Structure):
+    _fields_ = [("color", Tensor), ("termination_index", Tensor)]
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:dvr_status|int32_t|int64_t|const char\s*\*)\s*(dvr_\w+)\s*\(", text,
+                                 flags=re.M)))
+
+
+def lib():
+    l = ctypes.CDLL(LIB)
+    l.dvr_last_error.restype = ctypes.c_char_p
+    l.dvr_version.restype = ctypes.c_char_p
+    l.dvr_checkpoint_buffer_elements.restype = ctypes.c_int64
+    return l
+
+
+def tensor(shape, device, dtype=0, data=0x1000):
+    t = Tensor()
+    t.data = data
+    t.ndim = len(shape)
+    t.dtype = dtype
+    t.device = device
+    stride = 1
+    for d in reversed(range(len(shape))):
+        t.sizes[d] = shape[d]
+        t.strides[d] = stride
+        stride *= shape[d]
+    return t
+
+
+def valid_inputs(device):
+    i = Inputs()
+    i.screen_width, i.screen_height = 8, 6
+    i.volume_filter_mode = 1
+    i.volume = tensor((1, 5, 5, 5), device)
+    i.box_min_value[:] = [-0.5, -0.5, -0.5]
+    i.box_size_value[:] = [1, 1, 1]
+    i.camera_mode = 2
+    i.camera_matrix = tensor((1, 3, 3), device)
+    i.camera_fov_y_radians = 0.7
+    i.step_size_value = 0.01
+    i.tf_mode = 1
+    i.tf = tensor((1, 4, 4), device)
+    i.blend_mode = 0
+    i.blending_early_out = 1 - 1e-5
+    return i
+
+
+def test_header_functions_are_exported():
+    names = declared_functions()
+    assert len(names) >= 14, names
+    l = lib()
+    missing = [n for n in names if not hasattr(l, n)]
+    assert not missing, missing
+
+
+def test_library_identity():
+    l = lib()
+    assert l.dvr_use_double_precision() == 0
+    assert b"gfx950" in l.dvr_version()
+
+
+def test_host_tensors_are_rejected_loudly():
+    l = lib()
+    i = valid_inputs(-1)
+    o = Outputs(tensor((1, 6, 8, 4), -1), tensor((1, 6, 8), -1, dtype=1))
+    st = l.dvr_render_forward(ctypes.byref(i), ctypes.byref(o), None)
+    assert st == 3  # DVR_ERR_UNSUPPORTED
+    assert b"GPU only" in l.dvr_last_error()
+
+
+@pytest.mark.parametrize("mutate,code,msg", [
+    (lambda i: setattr(i, "tf", tensor((1, 4, 5), 0)), 1, b"tf must have 4 entries at dimension 2"),
+    (lambda i: setattr(i, "volume", tensor((1, 5, 5, 5), 0, dtype=1)), 1, b"volume must be of type torch.float32"),
+    (lambda i: setattr(i, "camera_matrix", tensor((2, 3, 3), 0)) or setattr(i, "tf", tensor((3, 4, 4), 0)), 1,
+     b"must be have the same batch size"),
+    (lambda i: setattr(i, "camera_mode", 7), 1, b"unknown camera enum value"),
+    (lambda i: setattr(i, "tf_mode", 6), 3, b"tf mode 6 is not supported"),
+    (lambda i: setattr(i, "step_size_value", 0.0), 1, b"step size must be positive"),
+    (lambda i: setattr(i, "volume", tensor((1, 1, 5, 5), 0)), 1, b"at least 2 voxels"),
+])
+def test_validation_messages(mutate, code, msg):
+    l = lib()
+    i = valid_inputs(0)
+    mutate(i)
+    o = Outputs(tensor((1, 6, 8, 4), 0), tensor((1, 6, 8), 0, dtype=1))
+    st = l.dvr_render_forward(ctypes.byref(i), ctypes.byref(o), None)
+    assert st == code, (st, l.dvr_last_error())
+    assert msg in l.dvr_last_error()
+
+
+def test_output_shape_mismatch_and_null_arguments():
+    l = lib()
+    i = valid_inputs(0)
+    o = Outputs(tensor((1, 6, 7, 4), 0), tensor((1, 6, 8), 0, dtype=1))
+    assert l.dvr_render_forward(ctypes.byref(i), ctypes.byref(o), None) == 1
+    assert b"color must have 8 entries at dimension 2" in l.dvr_last_error()
+    assert l.dvr_render_forward(None, ctypes.byref(o), None) == 1
+    assert l.dvr_checkpoint_buffer_elements(None, 16) == -1

@@ -1,0 +1,115 @@
+"""The torch front end (pytests/diffdvr/renderer.py:61-359) on the GPU.
+
+Gradients obtained with loss.backward() through diffdvr_amd.diffdvr.Renderer
+are compared with the FP64 oracle chain:
+  * TF and volume: oracle adjoint (exact replay, the GPU's sample counts);
+  * camera on a sphere (yaw, pitch, distance): oracle ray adjoint per pixel ->
+    oracle generateRays adjoint -> float64 torch autograd of viewport_from_sphere.
+The reference differentiates neither tmin nor the sample count w.r.t. the
+camera (renderer_kernels.cuh:378-412); the oracle chain does the same.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import oracle, scenes
+from gpu_helpers import dev, norm_err
+from diffdvr_amd import pyrenderer as pr
+from diffdvr_amd import diffdvr
+
+pytestmark = pytest.mark.gpu
+
+
+def base_settings(wl):
+    s = pr.RendererInputs()
+    s.screen_size = pr.int2(wl.width, wl.height)
+    s.volume = wl.volume
+    s.volume_filter_mode = pr.VolumeFilterMode.Trilinear
+    s.box_min = pr.real3(-0.5, -0.5, -0.5)
+    s.box_size = pr.real3(1, 1, 1)
+    s.step_size = wl.step
+    s.tf_mode = pr.TFMode.Linear
+    s.tf = wl.tf
+    s.blend_mode = pr.BlendMode.BeerLambert
+    s.camera_mode = pr.CameraMode.ReferenceFrame
+    return s
+
+
+def oracle_scene(wl, vp, tf, vol):
+    return dict(W=wl.width, H=wl.height, filter=1, camera=2, tf_mode=2, blend=0, early_out=1 - 1e-5,
+                fov_y=wl.fov_y, volume=vol.double().cpu().numpy(), box_min=np.full((1, 3), -0.5),
+                box_size=np.ones((1, 3)), step=np.full((1, 1, 1), wl.step), tf=tf.double().cpu().numpy(),
+                cam_matrix=vp.double().cpu().numpy())
+
+
+@pytest.mark.parametrize("adjoint_method", ["checkpoint", "inversion"])
+def test_tf_and_volume_gradients_through_autograd(adjoint_method):
+    wl = scenes.workload_c1().to(dev())
+    wl.width = wl.height = 48
+    vp = scenes.sphere_viewports([0.4, -0.3], [0.7, 2.9], 2.5).to(dev())
+    tf = wl.tf.clone().requires_grad_(True)
+    vol = wl.volume.clone().requires_grad_(True)
+    r = diffdvr.Renderer(base_settings(wl), optimize_tf=True, optimize_volume=True, adjoint_method=adjoint_method)
+    img = r(camera=vp, fov_y_radians=wl.fov_y, tf=tf, volume=vol)
+    rng = np.random.default_rng(3)
+    w = torch.as_tensor(rng.standard_normal(img.shape), dtype=torch.float32, device=dev())
+    (img * w).sum().backward()
+    torch.cuda.synchronize()
+    # oracle: same sample counts as the forward pass
+    s = oracle_scene(wl, vp, wl.tf, wl.volume)
+    color = torch.empty_like(img)
+    term = torch.empty(img.shape[:3], dtype=torch.int32, device=dev())
+    inp = base_settings(wl)
+    inp.camera = pr.CameraReferenceFrame(vp, wl.fov_y)
+    pr.Renderer.render_forward(inp, pr.RendererOutputs(color, term))
+    t = term.cpu().numpy()
+    c, _ = oracle.forward(s, frozen_term=t)
+    g = oracle.adjoint(s, c, t, w.double().cpu().numpy(), grads=("tf", "volume"), method=oracle.EXACT_REPLAY)
+    etf = norm_err(tf.grad.double().cpu().numpy(), g["tf"])
+    evol = norm_err(vol.grad.double().cpu().numpy()[:, 2:-2, 2:-2, 2:-2], g["volume"][:, 2:-2, 2:-2, 2:-2])
+    print(f"{adjoint_method}: tf grad err {etf:.2e}, interior volume grad err {evol:.2e}")
+    assert etf < 1e-4 and evol < 1e-4
+
+
+def test_camera_on_sphere_chain():
+    wl = scenes.workload_c1().to(dev())
+    wl.width, wl.height = 40, 32
+    cam = diffdvr.CameraOnASphere(pr.Orientation.Yp)
+    # cameras in the negative octant (yaw in (pi/2, pi), pitch > 0 with Orientation.Yp): every ray that
+    # hits the box enters through a min face, away from the kink of the clamped last trilinear cell
+    yaw = torch.tensor([[2.2], [2.7]], device=dev(), requires_grad=True)
+    pitch = torch.tensor([[0.35], [0.55]], device=dev(), requires_grad=True)
+    dist = torch.tensor([[2.4], [2.7]], device=dev(), requires_grad=True)
+    center = torch.zeros(2, 3, device=dev())
+    vp = cam(center, yaw, pitch, dist)
+    r = diffdvr.Renderer(base_settings(wl), optimize_camera=True)
+    img = r(camera=vp, fov_y_radians=wl.fov_y)
+    rng = np.random.default_rng(4)
+    w = torch.as_tensor(rng.standard_normal(img.shape), dtype=torch.float32, device=dev())
+    (img * w).sum().backward()
+    torch.cuda.synchronize()
+    # oracle chain in float64
+    rs, rd = pr.Camera.generate_rays(vp.detach(), wl.fov_y, wl.width, wl.height)
+    s = oracle_scene(wl, vp.detach(), wl.tf, wl.volume)
+    s["camera"] = 0
+    s["ray_start"], s["ray_dir"] = rs.double().cpu().numpy(), rd.double().cpu().numpy()
+    inp = base_settings(wl)
+    inp.camera_mode = pr.CameraMode.RayStartDir
+    inp.camera = pr.CameraPerPixelRays(rs, rd)
+    color = torch.empty_like(img)
+    term = torch.empty(img.shape[:3], dtype=torch.int32, device=dev())
+    pr.Renderer.render_forward(inp, pr.RendererOutputs(color, term))
+    t = term.cpu().numpy()
+    c, _ = oracle.forward(s, frozen_term=t)
+    g = oracle.adjoint(s, c, t, w.double().cpu().numpy(), grads=("camera",), method=oracle.EXACT_REPLAY)
+    avp = oracle.generate_rays_adjoint(vp.detach().double().cpu().numpy(), wl.height, wl.width, wl.fov_y,
+                                       g["ray_start"], g["ray_dir"])
+    y64, p64, d64 = (v.detach().double().cpu().requires_grad_(True) for v in (yaw, pitch, dist))
+    vp64 = pr.Camera.viewport_from_sphere(torch.zeros(2, 3, dtype=torch.float64), y64, p64, d64, pr.Orientation.Yp)
+    (vp64 * torch.as_tensor(avp)).sum().backward()
+    for got, want, name in ((yaw.grad, y64.grad, "yaw"), (pitch.grad, p64.grad, "pitch"), (dist.grad, d64.grad, "dist")):
+        e = norm_err(got.double().cpu().numpy(), want.numpy())
+        print(f"camera {name}: grad err {e:.2e}")
+        assert e < 1e-4, (name, e)
