@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--profile", action="store_true")
     p.add_argument("--checkpoint-interval", type=int, default=0)
+    p.add_argument("--pmc-probe", action="store_true",
+                   help="for rocprofv3 --pmc: 3 fused steps and one 256 MiB device copy (counter calibration)")
     return p.parse_args()
 
 
@@ -149,6 +151,17 @@ def main():
     def batched_step():
         adj_tf.zero_()
         pr.Renderer.render_fused(inp, outs, "l1", target, adj, checkpoint_interval=K)
+
+    if args.pmc_probe:
+        for _ in range(3):
+            batched_step()
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)   # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": src.numel() * 4,
+                          "samples_per_step": float(term.sum().item())}))
+        return
 
     # ---- batched fused path ------------------------------------------------------------------
     for _ in range(args.warmup):

@@ -67,6 +67,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks)
 struct Ray {
     f3 start, dir;
     f3 bmin, voxel, invVoxel;
+    f3 p0, dp;  // sample position in voxel units: p(t) = p0 + t * dp
     float step, t0, t1;
 };
 
@@ -171,6 +172,11 @@ __device__ __forceinline__ Ray setup_ray(const RenderParams& P, int x, int y, in
     r.start = mk3(float(start.x), float(start.y), float(start.z));
     r.dir = mk3(float(dir.x), float(dir.y), float(dir.z));
     r.step = P.st ? P.st[b * P.stSb + y * P.stSy + x * P.stSx] : P.stU;
+    {
+        const double vx = bsize.x / double(P.X - 1), vy = bsize.y / double(P.Y - 1), vz = bsize.z / double(P.Z - 1);
+        r.p0 = mk3(float((start.x - bmin.x) / vx), float((start.y - bmin.y) / vy), float((start.z - bmin.z) / vz));
+        r.dp = mk3(float(dir.x / vx), float(dir.y / vy), float(dir.z / vz));
+    }
     double t0, t1;
     slab(start, dir, bmin, bsize, t0, t1);
     t0 = t0 > 0.0 ? t0 : 0.0;
@@ -186,11 +192,11 @@ __device__ __forceinline__ Ray setup_ray(const RenderParams& P, int x, int y, in
     return r;
 }
 
-// volume-space sample position of parameter t: ((start + t dir) - bmin) / voxel
+// volume-space sample position of parameter t: ((start + t dir) - bmin) / voxel,
+// as one FMA per axis from per-ray constants rounded from double
 __device__ __forceinline__ f3 sample_pos(const Ray& r, float t)
 {
-    const f3 w = mk3(r.start.x + t * r.dir.x, r.start.y + t * r.dir.y, r.start.z + t * r.dir.z);
-    return mul3(sub3(w, r.bmin), r.invVoxel);
+    return mk3(fmaf(t, r.dp.x, r.p0.x), fmaf(t, r.dp.y, r.p0.y), fmaf(t, r.dp.z, r.p0.z));
 }
 
 // ---------------------------------------------------------------------------
@@ -265,6 +271,62 @@ __device__ __forceinline__ float fetch(const Vol& V, f3 p, Cell& c)
 }
 
 // ---------------------------------------------------------------------------
+// Fetch pipeline: the voxels of the NEXT sample are requested before the
+// current sample is shaded and blended, so the L1/L2 round trip overlaps a
+// sample's worth of arithmetic instead of stalling the wave (a wave of 8x8
+// rays enters a new cell on almost every step, so a plain cell cache issues
+// and waits for loads each sample).  Positions handed to request() must be
+// the ones later passed to value(): callers carry them from one iteration
+// to the next.
+//   start(V, p)   load the cell of the first position
+//   value(p)      interpolate at p (p lies in the current cell)
+//   request(V, pn) issue the loads of the cell of pn if it differs
+//   advance()     make pn's cell current (waits for the loads)
+// ---------------------------------------------------------------------------
+template <int FILTER>
+struct FetchPipe;
+
+template <>
+struct FetchPipe<1> {
+    int ix, iy, iz, jx, jy, jz;
+    bool chg;
+    float d[8], nd[8];
+    __device__ __forceinline__ void start(const Vol& V, f3 p)
+    {
+        ix = int(p.x); iy = int(p.y); iz = int(p.z);
+        cell_load(V, ix, iy, iz, d);
+        chg = false;
+    }
+    __device__ __forceinline__ float value(f3 p) const
+    {
+        return tri_lerp(d, p.x - float(ix), p.y - float(iy), p.z - float(iz));
+    }
+    __device__ __forceinline__ void request(const Vol& V, f3 pn)
+    {
+        jx = int(pn.x); jy = int(pn.y); jz = int(pn.z);
+        chg = (jx != ix) | (jy != iy) | (jz != iz);
+        if (chg) cell_load(V, jx, jy, jz, nd);
+    }
+    __device__ __forceinline__ void advance()
+    {
+        if (chg) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) d[k] = nd[k];
+            ix = jx; iy = jy; iz = jz;
+        }
+    }
+};
+
+template <>
+struct FetchPipe<0> {
+    float v, nv;
+    __device__ __forceinline__ void start(const Vol& V, f3 p) { v = V.p[nearest_offset(V, p)]; }
+    __device__ __forceinline__ float value(f3) const { return v; }
+    __device__ __forceinline__ void request(const Vol& V, f3 pn) { nv = V.p[nearest_offset(V, pn)]; }
+    __device__ __forceinline__ void advance() { v = nv; }
+};
+
+// ---------------------------------------------------------------------------
 // Transfer functions.  `tf` points at row 0 of this batch's R x C table with
 // row stride sr and column stride sc (global or LDS memory).
 // ---------------------------------------------------------------------------
@@ -311,6 +373,40 @@ __device__ __forceinline__ f4 tf_eval(const Tf& T, float density)
     }
 }
 
+// Absorption channel of the TF only (the transmittance recompute needs no colour).
+template <int TFM>
+__device__ __forceinline__ float tf_absorption(const Tf& T, float density)
+{
+    if constexpr (TFM == 0) {
+        return fclamp(density, 0.0f, 1.0f) * T.p[0];
+    } else if constexpr (TFM == 1) {
+        const float d = density * float(T.R) - 0.5f;
+        const int di = int(floorf(d));
+        const float df = d - float(di);
+        const float a = T.p[iclamp(di, 0, T.R - 1) * T.sr + 3 * T.sc];
+        const float c = T.p[iclamp(di + 1, 0, T.R - 1) * T.sr + 3 * T.sc];
+        return lerpf_(a, c, df);
+    } else if constexpr (TFM == 2) {
+        int i;
+        for (i = 0; i < T.R - 2; ++i)
+            if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
+        const float* a = T.p + i * T.sr;
+        const float* c = a + T.sr;
+        const float p0 = a[4 * T.sc], p1 = c[4 * T.sc];
+        const float d2 = fclamp(density, p0, p1);
+        const float f = (d2 - p0) / (p1 - p0);
+        return lerpf_(a[3 * T.sc], c[3 * T.sc], f);
+    } else {
+        float o = 0.0f;
+        for (int i = 0; i < T.R; ++i) {
+            const float* r = T.p + i * T.sr;
+            const float mu = r[4 * T.sc], s = r[5 * T.sc];
+            o += r[3 * T.sc] * expf(-(density - mu) * (density - mu) / (2.0f * s * s));
+        }
+        return o;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Blending (renderer_blending.cuh:54-61, 118-125)
 // ---------------------------------------------------------------------------
@@ -345,13 +441,27 @@ struct Acc {
     float r, g, b, A, T;
 };
 
+// Beer-Lambert: keep = exp(-x), alpha = 1 - exp(-x).  For x < 0.5, alpha is
+// its Taylor series to degree 8 (truncation < 2e-8 relative), free of the
+// cancellation of 1 - keep; for larger x, alpha = 1 - keep > 0.39 is well
+// conditioned.  Both forms are evaluated and selected without a branch, so
+// the fetches in flight around a sample are not split by divergent control flow.
 template <int BLEND>
 __device__ __forceinline__ void alpha_keep(float absorption, float step, float& alpha, float& keep)
 {
     if constexpr (BLEND == 0) {
         const float x = absorption * step;
         keep = expf(-x);
-        alpha = x < 0.5f ? -expm1f(-x) : 1.0f - keep;
+        const float xs = fminf(x, 0.5f);
+        float p = -1.0f / 40320.0f;
+        p = fmaf(p, xs, 1.0f / 5040.0f);
+        p = fmaf(p, xs, -1.0f / 720.0f);
+        p = fmaf(p, xs, 1.0f / 120.0f);
+        p = fmaf(p, xs, -1.0f / 24.0f);
+        p = fmaf(p, xs, 1.0f / 6.0f);
+        p = fmaf(p, xs, -0.5f);
+        p = fmaf(p, xs, 1.0f);
+        alpha = x < 0.5f ? p * xs : 1.0f - keep;
     } else {
         alpha = tmin_(1.0f, absorption * step);
         keep = 1.0f - alpha;

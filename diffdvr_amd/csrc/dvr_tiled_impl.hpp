@@ -58,21 +58,29 @@ template <int FILTER, int TFM, int BLEND>
 __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const Ray& r, float earlyOut, int frozen,
                                               Acc& acc, int& n, float* ck, int ckS, int K, int cap)
 {
-    Cell c;
-    cell_invalidate(c);
     acc = Acc{0.f, 0.f, 0.f, 0.f, 1.f};
     const float tEarly = 1.0f - earlyOut;
     int countdown = K, cidx = 0;
     int i = 0;
+    float t = r.t0;
+    f3 p = sample_pos(r, t);
+    FetchPipe<FILTER> pipe;
+    bool started = false;
     for (; i < kMaxSamples; ++i) {
-        const float t = r.t0 + float(i) * r.step;
         if (frozen >= 0) {
             if (i >= frozen) break;
         } else {
             if (t > r.t1) break;
             if (acc.T < tEarly) break;
         }
-        const float d = fetch<FILTER>(V, sample_pos(r, t), c);
+        if (!started) {
+            pipe.start(V, p);
+            started = true;
+        }
+        const float d = pipe.value(p);
+        const float tn = r.t0 + float(i + 1) * r.step;
+        const f3 pn = sample_pos(r, tn);
+        pipe.request(V, pn);
         blend_t<BLEND>(acc, tf_eval<TFM>(T, d), r.step);
         if (ck) {
             if (--countdown == 0) {
@@ -81,6 +89,9 @@ __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const R
                 ++cidx;
             }
         }
+        pipe.advance();
+        t = tn;
+        p = pn;
     }
     n = i;
 }
@@ -213,23 +224,29 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 const float t = r.t0 + float(i) * r.step;
                 const float d = fetch<FILTER>(V, sample_pos(r, t), cc);
                 float alpha, keep;
-                alpha_keep<BLEND>(tf_eval<TFM>(T, d).w, r.step, alpha, keep);
+                alpha_keep<BLEND>(tf_absorption<TFM>(T, d), r.step, alpha, keep);
                 Tr *= keep;
             }
         }
-        // recompute the segment forward, keep density and transmittance-before per sample
+        // recompute the segment forward (fetches pipelined one sample ahead),
+        // keep density and transmittance-before per sample
         {
-            Cell cc;
-            cell_invalidate(cc);
+            FetchPipe<FILTER> pipe;
+            f3 p = sample_pos(r, r.t0 + float(i0) * r.step);
+            pipe.start(V, p);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (j < cnt) {
-                    const float t = r.t0 + float(i0 + j) * r.step;
-                    dv[j] = fetch<FILTER>(V, sample_pos(r, t), cc);
+                    dv[j] = pipe.value(p);
+                    const f3 pn = sample_pos(r, r.t0 + float(i0 + j + 1) * r.step);
+                    const bool more = j + 1 < cnt;
+                    if (more) pipe.request(V, pn);
                     tin[j] = Tr;
                     float alpha, keep;
-                    alpha_keep<BLEND>(tf_eval<TFM>(T, dv[j]).w, r.step, alpha, keep);
+                    alpha_keep<BLEND>(tf_absorption<TFM>(T, dv[j]), r.step, alpha, keep);
                     Tr *= keep;
+                    if (more) pipe.advance();
+                    p = pn;
                 }
             }
         }
