@@ -1,29 +1,35 @@
 """Throughput of DiffDVR's differentiable ray marcher on one MI355X.
 
-Metric: forward+adjoint ray-samples/s of the batched, checkpointed, fused
-path (one launch: forward march, per-pixel L1 loss, checkpointed reverse march
-with an LDS-privatised TF gradient) against the plain per-view form (per view:
-reference-structured forward launch, torch loss gradient, reference-structured
-inversion adjoint launch).  A ray-sample is one blended march iteration;
-N_samples = sum of the termination indices of the forward pass, and the
-adjoint processes the same N.
+Metric: forward+adjoint ray-samples/s of the batched, checkpointed path -- ONE
+persistent launch renders every view, forms the image-loss gradient in
+registers and runs the checkpointed reverse march -- against the plain
+per-view form (per view: reference-structured forward launch, torch loss
+gradient, reference-structured inversion adjoint launch).  A ray-sample is one
+blended march iteration; N_samples = sum of the termination indices of the
+forward pass, and the adjoint processes the same N.
 
-Workload (default, the TF-gradient configuration): 128^3 float32 density of
-32 seeded Gaussian blobs, 64x4 texture TF (U[0,1]), 8 cameras at latitude 20
-degrees evenly spaced in longitude, radius 2.5, 256x256 images, step 0.25 voxel,
-trilinear, Beer-Lambert, L1 loss against an image rendered with a second
-seeded TF; gradient w.r.t. the 64x4 TF.  Synthetic data, identical on every host.
+Workloads (synthetic, seeded, identical on every host; diffdvr_amd/scenes.py):
+  c2 (default, the metric's configuration): 128^3 blobs, 64x4 texture TF,
+     8 views 256^2 at latitude 20, step 0.25 voxel, L1 vs a second TF, TF gradient
+  c3: 256^3 blobs + noise, identity TF x10, 64 Fibonacci views 256^2, step 0.5
+     voxel, L2 vs a second volume, gradient w.r.t. all voxels, K autotuned
+  c4: 128^3 blobs, 4-Gaussian TF, 512 cameras of sizes {64..512}^2, L2 vs images
+     from perturbed cameras, gradient w.r.t. latitude / longitude / radius
+  c5: 16 volumes {64..256}^3 with 4-32 cameras each at 128..512, linear TF per
+     volume, L2, density + TF gradient, HIP Adam, 100 steps captured as one HIP graph
 
-Prints ONE JSON line (rank 0).  Options:
-  --steps K --warmup W   timed / untimed steps (a step = one pass over the batch)
-  --gpus N               only N=1 is supported (one GPU per box)
-  --no-cpu-baseline      skip the CPU baseline leg
-  --profile              short run for rocprofv3 (no baseline, no plain leg)
+Prints ONE JSON line.  Options:
+  --workload c2|c3|c4|c5  --steps K --warmup W  --gpus N (only 1)
+  --no-cpu-baseline  --profile (short run for rocprofv3: no plain leg, no baseline)
+  --path persistent|fused (c2: persistent plan, or the single-batch fused kernel)
+  --checkpoint-interval K  --wave-width W  (0 = autotuned for c3/c5, defaults for c2/c4)
+  --pmc-probe (3 batched steps and one 256 MiB device copy, for counter calibration)
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -32,41 +38,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
-BYTES_PER_SAMPLE_TF = 64     # 32 B forward gathers + 32 B adjoint re-gathers (TF gradient privatised)
+# algorithmic bytes per ray-sample of the batched kernel: 32 B of trilinear corner
+# gathers in the forward march, 32 B again in the segment recompute, plus 32 B of
+# volume-gradient atomics (8 corners) or 32 B of corner reloads for the position
+# (camera) gradient
+BYTES_PER_SAMPLE = {"c2": 64, "c3": 96, "c4": 96, "c5": 96}
 METRIC = "fwd+adjoint ray-samples/s on 1 MI355X, batched vs plain, at 1e-4 rel grad err"
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    p.add_argument("--steps", type=int, default=0, help="timed steps (0 = per-workload default)")
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--profile", action="store_true")
+    p.add_argument("--path", default="persistent", choices=["persistent", "fused"])
     p.add_argument("--checkpoint-interval", type=int, default=0)
-    p.add_argument("--pmc-probe", action="store_true",
-                   help="for rocprofv3 --pmc: 3 fused steps and one 256 MiB device copy (counter calibration)")
+    p.add_argument("--wave-width", type=int, default=0)
+    p.add_argument("--graph-steps", type=int, default=100)
+    p.add_argument("--pmc-probe", action="store_true")
     return p.parse_args()
 
 
-def make_inputs(pr, wl, tf):
-    inp = pr.RendererInputs()
-    inp.screen_size = pr.int2(wl.width, wl.height)
-    inp.volume = wl.volume
-    inp.volume_filter_mode = pr.VolumeFilterMode.Trilinear
-    inp.box_min = pr.real3(-0.5, -0.5, -0.5)
-    inp.box_size = pr.real3(1.0, 1.0, 1.0)
-    inp.camera_mode = pr.CameraMode.ReferenceFrame
-    inp.camera = pr.CameraReferenceFrame(wl.viewports, wl.fov_y)
-    inp.step_size = wl.step
-    inp.tf_mode = pr.TFMode.Texture
-    inp.tf = tf
-    inp.blend_mode = pr.BlendMode.BeerLambert
-    return inp
-
-
 def traffic_from_profiles(name):
-    """HBM bytes per launch of the fused kernel from the committed PMC summary, if present."""
+    """HBM bytes per launch of the batched kernel from a committed PMC summary, if present."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -79,10 +76,81 @@ def traffic_from_profiles(name):
     return None
 
 
-def cpu_baseline(wl, target, color, term, adj_tf, scale):
+def timed(fn, steps, stream):
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def result_line(args, wl_name, value, ms, n_samples, config, kernel_ms, extra=None):
+    achieved = n_samples * BYTES_PER_SAMPLE[wl_name] / (kernel_ms * 1e-3)
+    r = {
+        "metric": METRIC, "value": value, "unit": "ray-samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": config,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": traffic_from_profiles(f"pmc_traffic_batched_{wl_name.upper()}.txt"),
+                     "kernel": "dvr::k_batched" if (wl_name != "c2" or args.path == "persistent")
+                     else "dvr::k_adjoint_tiled (fused)",
+                     "kernel_ms": kernel_ms, "bytes_per_sample": BYTES_PER_SAMPLE[wl_name]},
+    }
+    if extra:
+        r.update(extra)
+    return r
+
+
+# ---------------------------------------------------------------------------
+# plan construction shared by the workloads
+# ---------------------------------------------------------------------------
+def reference_frame_views(bt, cams_vp, sizes, vol_idx, tf_idx, fov, step, targets=None, grads=None):
+    views = []
+    for i, (w, h) in enumerate(sizes):
+        views.append(bt.BatchView(volume=vol_idx, tf=tf_idx, width=w, height=h, camera=cams_vp[i], fov_y=fov,
+                                  step=step, target=None if targets is None else targets[i],
+                                  camera_grad=None if grads is None else grads[i]))
+    return views
+
+
+def render_targets(bt, vols, tfs, views, tf_mode, filter_mode=1):
+    """Forward-only plan: images of (possibly different) volumes / cameras, outside any timing."""
+    import torch
+    plan = bt.BatchPlan([bt.BatchVolume(v) for v in vols], [bt.BatchTf(t) for t in tfs], views,
+                        volume_filter_mode=filter_mode, tf_mode=tf_mode, forward_only=True)
+    plan.run()
+    torch.cuda.synchronize()
+    return [v.color for v in views]
+
+
+# ---------------------------------------------------------------------------
+# C2 (the metric's configuration)
+# ---------------------------------------------------------------------------
+def make_inputs(pr, wl, tf, viewports=None):
+    inp = pr.RendererInputs()
+    inp.screen_size = pr.int2(wl.width, wl.height)
+    inp.volume = wl.volume
+    inp.volume_filter_mode = pr.VolumeFilterMode.Trilinear
+    inp.box_min = pr.real3(-0.5, -0.5, -0.5)
+    inp.box_size = pr.real3(1.0, 1.0, 1.0)
+    inp.camera_mode = pr.CameraMode.ReferenceFrame
+    inp.camera = pr.CameraReferenceFrame(wl.viewports if viewports is None else viewports, wl.fov_y)
+    inp.step_size = wl.step
+    inp.tf_mode = pr.TFMode.Texture
+    inp.tf = tf
+    inp.blend_mode = pr.BlendMode.BeerLambert
+    return inp
+
+
+def cpu_baseline_c2(wl, target, color, term, adj_tf, scale):
     """The FP32 OpenMP restatement of renderer_cpu.cpp on the whole workload
     (forward + L1 gradient + inversion adjoint with TF gradient), and the FP64
-    error of the GPU's fused result (same sample counts)."""
+    error of the GPU's batched result (same sample counts)."""
     import numpy as np
     from oracle import oracle
 
@@ -93,14 +161,12 @@ def cpu_baseline(wl, target, color, term, adj_tf, scale):
              step=np.full((1, 1, 1), wl.step), tf=wl.tf.double().cpu().numpy(),
              cam_matrix=wl.viewports.double().cpu().numpy())
     tg = target.double().cpu().numpy()
-    # timed: FP32 forward + loss gradient + FP32 inversion adjoint
     t0 = time.perf_counter()
     c32, t32 = oracle.forward(s, dtype=np.float32)
     adj = (np.sign(c32 - tg) * scale).astype(np.float32)
     oracle.adjoint(s, c32, t32, adj, grads=("tf",), dtype=np.float32)
     dt = time.perf_counter() - t0
     samples = float(t32.sum())
-    # accuracy of the GPU's fused result (FP64, the GPU's sample counts, the kernel's loss sign)
     c_gpu = color.double().cpu().numpy()
     t_gpu = term.cpu().numpy()
     c64, _ = oracle.forward(s, frozen_term=t_gpu)
@@ -116,41 +182,53 @@ def cpu_baseline(wl, target, color, term, adj_tf, scale):
     }, ferr, gerr
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: only --gpus 1 is supported; this build measures one MI355X "
-              f"(multi-GPU sharding is a design note, not a build target)", file=sys.stderr)
-        sys.exit(2)
-
+def bench_c2(args):
     import numpy as np
     import torch
     from diffdvr_amd import pyrenderer as pr
     from diffdvr_amd import scenes
+    from diffdvr_amd import batch as bt
 
     dev = torch.device("cuda:0")
     wl = scenes.workload_c2(device="cpu").to(dev)
     B, H, W = wl.B, wl.height, wl.width
-    K = args.checkpoint_interval
     stream = torch.cuda.current_stream()
+    steps = args.steps or 20
+    args.steps = steps
 
-    # target images (second seeded TF), outside any timing
     target = torch.empty(B, H, W, 4, device=dev)
     tterm = torch.empty(B, H, W, dtype=torch.int32, device=dev)
     pr.Renderer.render_forward(make_inputs(pr, wl, wl.target_tf), pr.RendererOutputs(target, tterm))
 
-    inp = make_inputs(pr, wl, wl.tf)
-    color = torch.empty(B, H, W, 4, device=dev)
-    term = torch.empty(B, H, W, dtype=torch.int32, device=dev)
     adj_tf = torch.zeros_like(wl.tf)
-    adj = pr.AdjointOutputs()
-    adj.has_tf_derivatives = True
-    adj.adj_tf = adj_tf
-    outs = pr.RendererOutputs(color, term)
+    if args.path == "persistent":
+        vol = wl.volume[0].contiguous()
+        tf = wl.tf[0]
+        color = torch.empty(B, H, W, 4, device=dev)
+        term = torch.empty(B, H, W, dtype=torch.int32, device=dev)
+        views = [bt.BatchView(volume=0, tf=0, width=W, height=H, camera=wl.viewports[b], fov_y=wl.fov_y,
+                              step=wl.step, color=color[b], termination=term[b], target=target[b])
+                 for b in range(B)]
+        plan = bt.BatchPlan([bt.BatchVolume(vol)], [bt.BatchTf(tf, adj_tf[0])], views, volume_filter_mode=1,
+                            tf_mode=1, blend_mode=0, loss="l1",
+                            checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8)
 
-    def batched_step():
-        adj_tf.zero_()
-        pr.Renderer.render_fused(inp, outs, "l1", target, adj, checkpoint_interval=K)
+        def batched_step():
+            plan.run(zero_gradients=True)
+        options = plan.options
+    else:
+        inp = make_inputs(pr, wl, wl.tf)
+        color = torch.empty(B, H, W, 4, device=dev)
+        term = torch.empty(B, H, W, dtype=torch.int32, device=dev)
+        adj = pr.AdjointOutputs()
+        adj.has_tf_derivatives = True
+        adj.adj_tf = adj_tf
+        outs = pr.RendererOutputs(color, term)
+
+        def batched_step():
+            adj_tf.zero_()
+            pr.Renderer.render_fused(inp, outs, "l1", target, adj, checkpoint_interval=args.checkpoint_interval)
+        options = {"checkpoint_interval": args.checkpoint_interval or 16}
 
     if args.pmc_probe:
         for _ in range(3):
@@ -161,59 +239,36 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"pmc_probe": True, "copy_bytes": src.numel() * 4,
                           "samples_per_step": float(term.sum().item())}))
-        return
+        return None
 
-    # ---- batched fused path ------------------------------------------------------------------
     for _ in range(args.warmup):
         batched_step()
     torch.cuda.synchronize()
     n_samples = float(term.sum().item())
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        batched_step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms_batched = ev0.elapsed_time(ev1) / args.steps
+    ms_batched = timed(batched_step, steps, stream)
     value = n_samples / (ms_batched * 1e-3)
-
-    # fused kernel alone (one launch per call), HIP events on the launch stream
-    k_ms = []
-    for _ in range(max(3, args.steps // 2)):
-        adj_tf.zero_()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        pr.Renderer.render_fused(inp, outs, "l1", target, adj, checkpoint_interval=K)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        k_ms.append(e0.elapsed_time(e1))
+    # the batched launch alone (one call = clear kernel + persistent kernel), HIP events on its stream
+    k_ms = [timed(batched_step, 1, stream) for _ in range(max(3, steps // 2))]
     kernel_ms = float(np.mean(k_ms))
-    achieved = n_samples * BYTES_PER_SAMPLE_TF / (kernel_ms * 1e-3)
 
-    result = {
-        "metric": METRIC, "value": value, "unit": "ray-samples/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_batched, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "C2: 128^3 blob volume, 64x4 texture TF gradient, 8 views 256x256, step 0.25 voxel, "
-                               "L1 loss, fused checkpointed fwd+adjoint",
-                   "views": B, "image": [H, W], "volume": list(wl.volume.shape[1:]), "samples_per_step": n_samples,
-                   "checkpoint_interval": K or 16},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic_from_profiles("pmc_traffic_fused_C2.txt"),
-                     "kernel": "dvr::k_adjoint_tiled (fused)", "kernel_ms": kernel_ms,
-                     "bytes_per_sample": BYTES_PER_SAMPLE_TF},
-    }
+    config = {"workload": "C2: 128^3 blob volume, 64x4 texture TF gradient, 8 views 256x256, step 0.25 voxel, "
+                          "L1 loss, one persistent fwd+loss+checkpointed-adjoint launch"
+                          if args.path == "persistent" else
+                          "C2: 128^3 blob volume, 64x4 texture TF gradient, 8 views 256x256, step 0.25 voxel, "
+                          "L1 loss, single-batch fused fwd+loss+checkpointed-adjoint launch",
+              "views": B, "image": [H, W], "volume": list(wl.volume.shape[1:]), "samples_per_step": n_samples,
+              "path": args.path, **options}
+    result = result_line(args, "c2", value, ms_batched, n_samples, config, kernel_ms)
+    if args.path == "fused":
+        result["roofline"]["traffic"] = traffic_from_profiles("pmc_traffic_fused_C2.txt")
 
     if not args.profile:
-        # ---- plain per-view path (reference structure) ----------------------------------------
-        views = []
+        views_p = []
         for b in range(B):
-            ib = make_inputs(pr, wl, wl.tf)
-            ib.camera = pr.CameraReferenceFrame(wl.viewports[b:b + 1].contiguous(), wl.fov_y)
+            ib = make_inputs(pr, wl, wl.tf, wl.viewports[b:b + 1].contiguous())
             cb = torch.empty(1, H, W, 4, device=dev)
             tb = torch.empty(1, H, W, dtype=torch.int32, device=dev)
-            views.append((ib, cb, tb, target[b:b + 1].contiguous()))
+            views_p.append((ib, cb, tb, target[b:b + 1].contiguous()))
         padj_tf = torch.zeros_like(wl.tf)
         padj = pr.AdjointOutputs()
         padj.has_tf_derivatives = True
@@ -223,7 +278,7 @@ def main():
 
         def plain_step():
             padj_tf.zero_()
-            for ib, cb, tb, tgb in views:
+            for ib, cb, tb, tgb in views_p:
                 o = pr.RendererOutputs(cb, tb)
                 pr.Renderer.render_forward(ib, o, path="plain")
                 g = torch.sign(cb - tgb) / n_all
@@ -231,27 +286,290 @@ def main():
 
         for _ in range(max(1, args.warmup)):
             plain_step()
-        torch.cuda.synchronize()
-        p_steps = max(3, args.steps // 4)
-        ev0.record(stream)
-        for _ in range(p_steps):
-            plain_step()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        ms_plain = ev0.elapsed_time(ev1) / p_steps
+        ms_plain = timed(plain_step, max(3, steps // 4), stream)
         plain_value = n_samples / (ms_plain * 1e-3)
+        batched_step()
+        torch.cuda.synchronize()
         plain_err = float(torch.linalg.norm(padj_tf - adj_tf) / torch.linalg.norm(adj_tf))
         result["plain"] = {"value": plain_value, "ms_per_step": ms_plain, "speedup_batched_vs_plain": value / plain_value,
                            "tf_grad_rel_diff_vs_batched": plain_err}
-
         if not args.no_cpu_baseline:
-            # the batched buffers hold the last fused step: colour, termination, TF gradient
-            batched_step()
-            torch.cuda.synchronize()
-            base, ferr, gerr = cpu_baseline(wl, target, color, term, adj_tf, 1.0 / n_all)
+            base, ferr, gerr = cpu_baseline_c2(wl, target, color, term, adj_tf, 1.0 / n_all)
             result["cpu_baseline"] = base
             result["accuracy"] = {"forward_rel_err_vs_fp64": ferr, "tf_grad_rel_err_vs_fp64": gerr}
-    print(json.dumps(result))
+    return result
+
+
+# ---------------------------------------------------------------------------
+# C3: volume gradient
+# ---------------------------------------------------------------------------
+def bench_c3(args):
+    import numpy as np
+    import torch
+    from diffdvr_amd import pyrenderer as pr
+    from diffdvr_amd import scenes
+    from diffdvr_amd import batch as bt
+
+    dev = torch.device("cuda:0")
+    wl = scenes.workload_c3(device="cpu").to(dev)
+    B, H, W = wl.B, wl.height, wl.width
+    steps = args.steps or 5
+    args.steps = steps
+    stream = torch.cuda.current_stream()
+    vol = wl.volume[0].contiguous()
+    tf = wl.tf[0].contiguous()
+    vps = [wl.viewports[b] for b in range(B)]
+    tviews = reference_frame_views(bt, vps, [(W, H)] * B, 0, 0, wl.fov_y, wl.step)
+    targets = render_targets(bt, [wl.target_volume[0].contiguous()], [tf], tviews, 0)
+    adj_vol = torch.zeros_like(vol)
+    views = reference_frame_views(bt, vps, [(W, H)] * B, 0, 0, wl.fov_y, wl.step, targets=targets)
+    plan = bt.BatchPlan([bt.BatchVolume(vol, grad=adj_vol)], [bt.BatchTf(tf)], views, volume_filter_mode=1,
+                        tf_mode=0, blend_mode=0, loss="l2",
+                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8)
+    tuned = None
+    if not args.checkpoint_interval and not args.wave_width:
+        tuned = plan.autotune(iterations=1)
+
+    def step():
+        plan.run(zero_gradients=True)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    n_samples = float(sum(v.termination.sum().item() for v in views))
+    ms = timed(step, steps, stream)
+    value = n_samples / (ms * 1e-3)
+    config = {"workload": "C3: 256^3 blob volume + noise 0.02, identity TF x10, 64 Fibonacci views 256x256, "
+                          "step 0.5 voxel, L2 vs a second volume, gradient w.r.t. all voxels, K autotuned",
+              "views": B, "image": [H, W], "samples_per_step": n_samples, **plan.options}
+    result = result_line(args, "c3", value, ms, n_samples, config, ms,
+                         {"autotuned": tuned is not None})
+    if not args.profile:
+        inp = pr.RendererInputs()
+        inp.screen_size = pr.int2(W, H)
+        inp.volume = wl.volume
+        inp.volume_filter_mode = pr.VolumeFilterMode.Trilinear
+        inp.box_min = pr.real3(-0.5, -0.5, -0.5)
+        inp.box_size = pr.real3(1, 1, 1)
+        inp.camera_mode = pr.CameraMode.ReferenceFrame
+        inp.step_size = wl.step
+        inp.tf_mode = pr.TFMode.Identity
+        inp.tf = wl.tf
+        inp.blend_mode = pr.BlendMode.BeerLambert
+        padj_vol = torch.zeros_like(wl.volume)
+        padj = pr.AdjointOutputs()
+        padj.has_volume_derivatives = True
+        padj.adj_volume = padj_vol
+        n_all = float(B * H * W * 4)
+        per = []
+        for b in range(B):
+            cb = torch.empty(1, H, W, 4, device=dev)
+            tb = torch.empty(1, H, W, dtype=torch.int32, device=dev)
+            per.append((pr.CameraReferenceFrame(wl.viewports[b:b + 1].contiguous(), wl.fov_y), cb, tb,
+                        targets[b].unsqueeze(0)))
+
+        def plain_step():
+            padj_vol.zero_()
+            for cam, cb, tb, tgb in per:
+                inp.camera = cam
+                o = pr.RendererOutputs(cb, tb)
+                pr.Renderer.render_forward(inp, o, path="plain")
+                pr.Renderer.render_adjoint(inp, o, 2.0 * (cb - tgb) / n_all, padj, method="inversion", path="plain")
+
+        plain_step()
+        ms_plain = timed(plain_step, max(2, steps // 2), stream)
+        step()
+        torch.cuda.synchronize()
+        diff = float(torch.linalg.norm(padj_vol[0] - adj_vol) / torch.linalg.norm(adj_vol))
+        result["plain"] = {"value": n_samples / (ms_plain * 1e-3), "ms_per_step": ms_plain,
+                           "speedup_batched_vs_plain": ms_plain / ms, "vol_grad_rel_diff_vs_batched": diff}
+    return result
+
+
+# ---------------------------------------------------------------------------
+# C4: camera gradients of 512 cameras of uneven size
+# ---------------------------------------------------------------------------
+def bench_c4(args):
+    import numpy as np
+    import torch
+    from diffdvr_amd import pyrenderer as pr
+    from diffdvr_amd import scenes
+    from diffdvr_amd import batch as bt
+
+    dev = torch.device("cuda:0")
+    wl = scenes.workload_c4()
+    cams, tcams = wl.cameras[0], wl.target_cameras[0]
+    N = len(cams.sizes)
+    steps = args.steps or 5
+    args.steps = steps
+    stream = torch.cuda.current_stream()
+    vol = wl.volumes[0].to(dev).contiguous()
+    tf = wl.tfs[0].to(dev).contiguous()
+    step_w = wl.steps[0]
+    tvp = tcams.viewports(dev)
+    tviews = reference_frame_views(bt, [tvp[i] for i in range(N)], cams.sizes, 0, 0, wl.fov_y, step_w)
+    targets = render_targets(bt, [vol], [tf], tviews, 3)
+    del tviews
+    lat = cams.lat.to(dev).float().requires_grad_(True)
+    lon = cams.lon.to(dev).float().requires_grad_(True)
+    rad = cams.radius.to(dev).float().requires_grad_(True)
+    vp_buf = torch.zeros(N, 3, 3, device=dev)
+    g_vp = torch.zeros(N, 3, 3, device=dev)
+    views = reference_frame_views(bt, [vp_buf[i] for i in range(N)], cams.sizes, 0, 0, wl.fov_y, step_w,
+                                  targets=targets, grads=[g_vp[i] for i in range(N)])
+    loss = torch.zeros(1, device=dev)
+    plan = bt.BatchPlan([bt.BatchVolume(vol)], [bt.BatchTf(tf)], views, volume_filter_mode=1, tf_mode=3,
+                        blend_mode=0, loss="l2", loss_out=loss,
+                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8)
+
+    def step():
+        for p in (lat, lon, rad):
+            p.grad = None
+        vp = pr.Camera.viewport_from_sphere(torch.zeros(N, 3, device=dev), lon, lat, rad, pr.Orientation.Yp)
+        vp_buf.copy_(vp.detach())
+        plan.run(zero_gradients=True)
+        vp.backward(g_vp)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    n_samples = float(sum(v.termination.sum().item() for v in views))
+    ms = timed(step, steps, stream)
+    k_ms = float(np.mean([timed(lambda: plan.run(True), 1, stream) for _ in range(3)]))
+    value = n_samples / (ms * 1e-3)
+    config = {"workload": "C4: 128^3 blob volume, 4-component Gaussian TF, 512 sphere cameras of sizes "
+                          "{64,128,256,512}^2, step 0.5 voxel, L2 vs perturbed-camera images, gradient w.r.t. "
+                          "latitude/longitude/radius, one persistent launch",
+              "cameras": N, "rays": int(sum(w * h for w, h in cams.sizes)), "samples_per_step": n_samples,
+              **plan.options}
+    result = result_line(args, "c4", value, ms, n_samples, config, k_ms)
+    if not args.profile:
+        inp = pr.RendererInputs()
+        inp.volume = vol.unsqueeze(0)
+        inp.volume_filter_mode = pr.VolumeFilterMode.Trilinear
+        inp.box_min = pr.real3(-0.5, -0.5, -0.5)
+        inp.box_size = pr.real3(1, 1, 1)
+        inp.camera_mode = pr.CameraMode.RayStartDir
+        inp.step_size = step_w
+        inp.tf_mode = pr.TFMode.Gaussian
+        inp.tf = tf.unsqueeze(0)
+        inp.blend_mode = pr.BlendMode.BeerLambert
+        n_all = float(sum(w * h for w, h in cams.sizes) * 4)
+        bufs = []
+        for i, (w, h) in enumerate(cams.sizes):
+            rs = torch.empty(1, h, w, 3, device=dev)
+            rd = torch.empty(1, h, w, 3, device=dev)
+            ars = torch.zeros(1, h, w, 3, device=dev)
+            ard = torch.zeros(1, h, w, 3, device=dev)
+            cb = torch.empty(1, h, w, 4, device=dev)
+            tb = torch.empty(1, h, w, dtype=torch.int32, device=dev)
+            bufs.append((w, h, rs, rd, ars, ard, cb, tb, targets[i].unsqueeze(0)))
+        pg_vp = torch.zeros(N, 3, 3, device=dev)
+
+        def plain_step():
+            for p in (lat, lon, rad):
+                p.grad = None
+            vp = pr.Camera.viewport_from_sphere(torch.zeros(N, 3, device=dev), lon, lat, rad, pr.Orientation.Yp)
+            vpd = vp.detach()
+            for i, (w, h, rs, rd, ars, ard, cb, tb, tgb) in enumerate(bufs):
+                v1 = vpd[i:i + 1]
+                pr._n.generate_rays(v1, wl.fov_y, w, h, rs, rd)
+                inp.screen_size = pr.int2(w, h)
+                inp.camera = pr.CameraPerPixelRays(rs, rd)
+                o = pr.RendererOutputs(cb, tb)
+                pr.Renderer.render_forward(inp, o, path="plain")
+                a = pr.AdjointOutputs()
+                a.has_camera_derivatives = True
+                ars.zero_()
+                ard.zero_()
+                a.adj_camera_ray_start = ars
+                a.adj_camera_ray_dir = ard
+                pr.Renderer.render_adjoint(inp, o, 2.0 * (cb - tgb) / n_all, a, method="inversion", path="plain")
+                pr._n.generate_rays_adjoint(v1, wl.fov_y, w, h, ars, ard, pg_vp[i:i + 1])
+            vp.backward(pg_vp)
+
+        plain_step()
+        ms_plain = timed(plain_step, 2, stream)
+        result["plain"] = {"value": n_samples / (ms_plain * 1e-3), "ms_per_step": ms_plain,
+                           "speedup_batched_vs_plain": ms_plain / ms}
+    return result
+
+
+# ---------------------------------------------------------------------------
+# C5: multi-volume optimisation loop, 100 steps as one HIP graph
+# ---------------------------------------------------------------------------
+def bench_c5(args):
+    import numpy as np
+    import torch
+    from diffdvr_amd import scenes
+    from diffdvr_amd import batch as bt
+
+    dev = torch.device("cuda:0")
+    wl = scenes.workload_c5()
+    G = args.graph_steps
+    reps = args.steps or 1
+    args.steps = reps * G
+    stream = torch.cuda.current_stream()
+    vols = [v.to(dev).contiguous() for v in wl.volumes]
+    tgts = [v.to(dev).contiguous() for v in wl.target_volumes]
+    tfs = [t.to(dev).contiguous() for t in wl.tfs]
+    vps, sizes, vidx, steps_w = [], [], [], []
+    for k, cams in enumerate(wl.cameras):
+        vp = cams.viewports(dev)
+        for i in range(len(cams.sizes)):
+            vps.append(vp[i])
+            sizes.append(cams.sizes[i])
+            vidx.append(k)
+            steps_w.append(wl.steps[k])
+
+    def views_for(targets=None):
+        return [bt.BatchView(volume=vidx[i], tf=vidx[i], width=sizes[i][0], height=sizes[i][1], camera=vps[i],
+                             fov_y=wl.fov_y, step=steps_w[i], target=None if targets is None else targets[i])
+                for i in range(len(vps))]
+
+    targets = render_targets(bt, tgts, tfs, views_for(), 2)
+    g_vols = [torch.zeros_like(v) for v in vols]
+    g_tfs = [torch.zeros_like(t) for t in tfs]
+    views = views_for(targets)
+    loss = torch.zeros(1, device=dev)
+    plan = bt.BatchPlan([bt.BatchVolume(v, grad=g) for v, g in zip(vols, g_vols)],
+                        [bt.BatchTf(t, g) for t, g in zip(tfs, g_tfs)], views, volume_filter_mode=1, tf_mode=2,
+                        blend_mode=0, loss="l2", loss_out=loss,
+                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8)
+    tuned = None
+    if not args.checkpoint_interval and not args.wave_width:
+        tuned = plan.autotune(iterations=1)
+    adam = bt.Adam([(v, g, None, (0.0, 1.0)) for v, g in zip(vols, g_vols)] +
+                   [(t, g, 4, (0.0, 1e3)) for t, g in zip(tfs, g_tfs)], lr=1e-2)
+    plan.run()
+    torch.cuda.synchronize()
+    loss0 = float(loss.item())
+    graph = bt.GraphedSteps(plan, adam, G)
+    torch.cuda.synchronize()
+    ms_graph = timed(graph.replay, reps, stream)
+    n_samples = float(sum(v.termination.sum().item() for v in views))  # last step's sample count
+    ms = ms_graph / G
+    value = n_samples / (ms * 1e-3)
+    k_ms = float(np.mean([timed(lambda: plan.run(True), 1, stream) for _ in range(2)]))
+    config = {"workload": "C5: 16 blob volumes {64..256}^3, 4-32 cameras each at 128..512, linear TF per volume, "
+                          "step 0.5 voxel, L2 vs second volumes, density + TF gradient, HIP Adam, "
+                          f"{G} steps captured as one HIP graph",
+              "volumes": len(vols), "views": len(views), "rays": int(sum(w * h for w, h in sizes)),
+              "samples_per_step": n_samples, "graph_steps": G, "loss_first": loss0,
+              "loss_last": float(loss.item()), "adam_steps": adam.step_count, **plan.options}
+    return result_line(args, "c5", value, ms, n_samples, config, k_ms, {"autotuned": tuned is not None})
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: only --gpus 1 is supported; this build measures one MI355X "
+              f"(multi-GPU sharding is a design note, not a build target)", file=sys.stderr)
+        sys.exit(2)
+    fn = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload]
+    result = fn(args)
+    if result is not None:
+        print(json.dumps(result))
 
 
 if __name__ == "__main__":

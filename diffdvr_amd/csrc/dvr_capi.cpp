@@ -19,30 +19,21 @@
 #include "../../include/dvr.h"
 #include "dvr_launch.hpp"
 #include "dvr_params.hpp"
+#include "dvr_host_util.hpp"
+
+std::string& dvr_host::last_error()
+{
+    static thread_local std::string e;
+    return e;
+}
 
 namespace {
 
-thread_local std::string g_error;
-
-struct Fail {
-    dvr_status code;
-};
-
-[[noreturn]] void fail(dvr_status code, const char* fmt, ...)
-{
-    char buf[1024];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_error = buf;
-    throw Fail{code};
-}
-
-void check_hip(hipError_t e, const char* what)
-{
-    if (e != hipSuccess) fail(DVR_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
-}
+using dvr_host::Fail;
+using dvr_host::fail;
+using dvr_host::check_hip;
+using dvr_host::guarded;
+using dvr_host::DeviceGuard;
 
 const char* dtype_name(int d) { return d == DVR_FLOAT32 ? "torch.float32" : (d == DVR_INT32 ? "torch.int32" : "?"); }
 
@@ -395,19 +386,6 @@ void* workspace(int device, size_t bytes, hipStream_t stream)
     return w.ptr;
 }
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        check_hip(hipGetDevice(&prev), "hipGetDevice");
-        if (prev != dev) check_hip(hipSetDevice(dev), "hipSetDevice");
-    }
-    ~DeviceGuard()
-    {
-        int cur = -1;
-        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
-    }
-};
 
 bool tiled_ok(const Checked& c) { return (int64_t)c.P.R * c.P.C <= dvr::kMaxTfLds; }
 
@@ -430,19 +408,17 @@ int checkpoint_cap(const Checked& c, int K)
 
 int default_k(const Checked&) { return 16; }
 
-template <class F> dvr_status guarded(F&& f)
+// Checkpoint interval I in {8, 16, 32, 64}: register segments of K = min(I, 32)
+// samples, a checkpoint every I / K segments.
+dvr::CkptParams make_ckpt(const Checked& c, int interval)
 {
-    try {
-        g_error.clear();
-        f();
-        return DVR_OK;
-    } catch (const Fail& e) {
-        return e.code;
-    } catch (const std::exception& e) {
-        g_error = e.what();
-        return DVR_ERR_HIP;
-    }
+    if (interval != 8 && interval != 16 && interval != 32 && interval != 64)
+        fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 8, 16, 32 or 64");
+    dvr::CkptParams CK{nullptr, std::min(interval, 32), checkpoint_cap(c, interval), 1};
+    CK.every = interval / CK.K;
+    return CK;
 }
+
 
 void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forward_options* opt, void* stream)
 {
@@ -457,7 +433,7 @@ void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forwar
         fail(DVR_ERR_UNSUPPORTED, "tiled path needs R*C <= %d transfer-function entries", dvr::kMaxTfLds);
     if (K > 0) {
         if (path != DVR_PATH_TILED) fail(DVR_ERR_UNSUPPORTED, "checkpoints are written by the tiled forward only");
-        if (K != 8 && K != 16 && K != 32) fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 8, 16 or 32");
+        make_ckpt(c, K);  // validates K
         const dvr_tensor& ck = opt->checkpoints;
         const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * checkpoint_cap(c, K) * dvr::kBlock;
         if (!given(ck) || ck.dtype != DVR_FLOAT32 || ck.device != c.device)
@@ -471,10 +447,10 @@ void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forwar
     if (path == DVR_PATH_PLAIN) {
         check_hip(dvr::launch_forward_plain(c.P, c.filter, c.tfm, c.blend, s), "forward (plain) launch");
     } else {
-        dvr::CkptParams CK{nullptr, K > 0 ? K : 16, 0};
+        dvr::CkptParams CK{nullptr, 16, 0, 1};
         if (K > 0) {
+            CK = make_ckpt(c, K);
             CK.buf = (float*)opt->checkpoints.data;
-            CK.cap = checkpoint_cap(c, K);
         }
         check_hip(dvr::launch_forward_tiled(c.P, CK, c.filter, c.tfm, c.blend, s), "forward (tiled) launch");
     }
@@ -500,8 +476,7 @@ void do_adjoint(const dvr_inputs* inputs, const dvr_outputs* fwd, const dvr_tens
     if (!tiled_ok(c)) fail(DVR_ERR_UNSUPPORTED, "checkpointed adjoint needs R*C <= %d", dvr::kMaxTfLds);
     int K = opt ? opt->checkpoint_interval : 0;
     if (K == 0) K = default_k(c);
-    if (K != 8 && K != 16 && K != 32) fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 8, 16 or 32");
-    dvr::CkptParams CK{nullptr, K, checkpoint_cap(c, K)};
+    dvr::CkptParams CK = make_ckpt(c, K);
     const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * CK.cap * dvr::kBlock;
     bool fromFwd = false;
     if (opt && given(opt->checkpoints)) {
@@ -551,10 +526,9 @@ void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* lo
     }
     if (!tiled_ok(c)) fail(DVR_ERR_UNSUPPORTED, "fused rendering needs R*C <= %d", dvr::kMaxTfLds);
     if (K == 0) K = default_k(c);
-    if (K != 8 && K != 16 && K != 32) fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 8, 16 or 32");
+    dvr::CkptParams CK = make_ckpt(c, K);
     DeviceGuard g(c.device);
     hipStream_t s = (hipStream_t)stream;
-    dvr::CkptParams CK{nullptr, K, checkpoint_cap(c, K)};
     const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * CK.cap * dvr::kBlock;
     CK.buf = (float*)workspace(c.device, size_t(need) * sizeof(float), s);
     check_hip(dvr::launch_adjoint_tiled(c.P, A, CK, L, true, false, c.filter, c.tfm, c.blend, s), "fused launch");
@@ -567,7 +541,7 @@ void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* lo
 // ---------------------------------------------------------------------------
 extern "C" {
 
-const char* dvr_last_error(void) { return g_error.c_str(); }
+const char* dvr_last_error(void) { return dvr_host::last_error().c_str(); }
 int32_t dvr_use_double_precision(void) { return 0; }
 const char* dvr_version(void) { return "dvr-mi355x 0.1 (gfx950)"; }
 

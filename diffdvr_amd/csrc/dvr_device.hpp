@@ -37,6 +37,25 @@ __device__ __forceinline__ float tmax_(float a, float b) { return a > b ? a : b;
 __device__ __forceinline__ float fclamp(float f, float a, float b) { return fmaxf(a, fminf(f, b)); }
 __device__ __forceinline__ int iclamp(int f, int a, int b) { return max(a, min(f, b)); }
 
+// Adjoints of normalize and cross as the reference forms them
+// (renderer_adjoint.cuh:28-65).
+// adjNormalize (renderer_adjoint.cuh:28-45)
+__device__ __forceinline__ f3 adj_normalize(f3 in, f3 adj_out)
+{
+    const float len = sqrtf(dot3(in, in));
+    f3 adj_in = scl3(adj_out, 1.0f / len);
+    const float adj_len = dot3(adj_out, scl3(in, -1.0f / (len * len)));
+    return add3(adj_in, scl3(in, adj_len / len));
+}
+
+// adjCross (renderer_adjoint.cuh:47-65)
+__device__ __forceinline__ void adj_cross(f3 a, f3 b, f3 c, f3& aa, f3& ab)
+{
+    aa.y += b.z * c.x; ab.z += a.y * c.x; aa.z -= b.y * c.x; ab.y -= a.z * c.x;
+    aa.z += b.x * c.y; ab.x += a.z * c.y; aa.x -= b.z * c.y; ab.z -= a.x * c.y;
+    aa.x += b.y * c.z; ab.y += a.x * c.z; aa.y -= b.x * c.z; ab.x -= a.y * c.z;
+}
+
 // ---------------------------------------------------------------------------
 // Wave-level helpers (wave64)
 // ---------------------------------------------------------------------------
@@ -95,6 +114,30 @@ __device__ __forceinline__ d3 dnormalize(d3 v)
     return dk3(v.x * inv, v.y * inv, v.z * inv);
 }
 
+// InverseViewMatrix camera: unproject NDC (fx, fy) at depths 0 and 0.9
+// (renderer_camera.cuh:28-44, renderer_utils.cuh:53-58).
+__device__ __forceinline__ void camera_inverse_view(const double m[16], int x, int y, int W, int H, d3& start, d3& dir)
+{
+    const double fx = 2.0 * (x + 0.5) / double(W) - 1.0;
+    const double fy = 2.0 * (y + 0.5) / double(H) - 1.0;
+    start = matmul_proj_d(m, fx, fy, 0.0);
+    const d3 end = matmul_proj_d(m, fx, fy, 0.9);
+    dir = dnormalize(dk3(end.x - start.x, end.y - start.y, end.z - start.z));
+}
+
+// ReferenceFrame camera: viewport rows eye, right, up; front = up x right
+// (renderer_camera.cuh:46-61).
+__device__ __forceinline__ void camera_reference_frame(d3 eye, d3 right, d3 up, double tanX, double tanY, int x, int y,
+                                                       int W, int H, d3& start, d3& dir)
+{
+    const d3 front = dcross(up, right);
+    const double fx = (2.0 * (x + 0.5) / double(W) - 1.0) * tanX;
+    const double fy = (2.0 * (y + 0.5) / double(H) - 1.0) * tanY;
+    start = eye;
+    dir = dnormalize(dk3(front.x + fx * right.x + fy * up.x, front.y + fx * right.y + fy * up.y,
+                         front.z + fx * right.z + fy * up.z));
+}
+
 // Camera from the standard parameter block (renderer_camera.cuh:12-61).
 __device__ __forceinline__ void camera_ray(const RenderParams& P, int x, int y, int b, d3& start, d3& dir)
 {
@@ -115,22 +158,13 @@ __device__ __forceinline__ void camera_ray(const RenderParams& P, int x, int y, 
 #pragma unroll
             for (int k = 0; k < 16; ++k) m[k] = P.cmU[k];
         }
-        const double fx = 2.0 * (x + 0.5) / double(P.W) - 1.0;
-        const double fy = 2.0 * (y + 0.5) / double(P.H) - 1.0;
-        start = matmul_proj_d(m, fx, fy, 0.0);
-        const d3 end = matmul_proj_d(m, fx, fy, 0.9);
-        dir = dnormalize(dk3(end.x - start.x, end.y - start.y, end.z - start.z));
+        camera_inverse_view(m, x, y, P.W, P.H, start, dir);
     } else {  // ReferenceFrame
         const float* c = P.cm + b * P.cmSb;
         const d3 eye = dk3(c[0], c[P.cmSc], c[2 * P.cmSc]);
         const d3 right = dk3(c[P.cmSr], c[P.cmSr + P.cmSc], c[P.cmSr + 2 * P.cmSc]);
         const d3 up = dk3(c[2 * P.cmSr], c[2 * P.cmSr + P.cmSc], c[2 * P.cmSr + 2 * P.cmSc]);
-        const d3 front = dcross(up, right);
-        const double fx = (2.0 * (x + 0.5) / double(P.W) - 1.0) * P.tanX;
-        const double fy = (2.0 * (y + 0.5) / double(P.H) - 1.0) * P.tanY;
-        start = eye;
-        dir = dnormalize(dk3(front.x + fx * right.x + fy * up.x, front.y + fx * right.y + fy * up.y,
-                             front.z + fx * right.z + fy * up.z));
+        camera_reference_frame(eye, right, up, P.tanX, P.tanY, x, y, P.W, P.H, start, dir);
     }
 }
 
@@ -147,33 +181,19 @@ __device__ __forceinline__ void slab(d3 start, d3 dir, d3 bmin, d3 bsize, double
     t1 = mn(mn(mx(a1, a2), mx(a3, a4)), mx(a5, a6));
 }
 
-__device__ __forceinline__ Ray setup_ray(const RenderParams& P, int x, int y, int b)
+// Ray of a camera ray through the box [bmin, bmin + bsize] of an X x Y x Z
+// volume (renderer_kernels.cuh:31-48), rounded from double.
+__device__ __forceinline__ Ray make_ray(d3 start, d3 dir, d3 bmin, d3 bsize, int X, int Y, int Z, float step)
 {
     Ray r;
-    d3 bmin, bsize;
-    if (P.bmin) {
-        const float* p = P.bmin + b * P.bminSb;
-        bmin = dk3(p[0], p[P.bminSc], p[2 * P.bminSc]);
-    } else {
-        bmin = dk3(P.bminU[0], P.bminU[1], P.bminU[2]);
-    }
-    if (P.bsize) {
-        const float* p = P.bsize + b * P.bsizeSb;
-        bsize = dk3(p[0], p[P.bsizeSc], p[2 * P.bsizeSc]);
-    } else {
-        bsize = dk3(P.bsizeU[0], P.bsizeU[1], P.bsizeU[2]);
-    }
     r.bmin = mk3(float(bmin.x), float(bmin.y), float(bmin.z));
-    r.voxel = div3(mk3(float(bsize.x), float(bsize.y), float(bsize.z)),
-                   mk3(float(P.X - 1), float(P.Y - 1), float(P.Z - 1)));
+    r.voxel = div3(mk3(float(bsize.x), float(bsize.y), float(bsize.z)), mk3(float(X - 1), float(Y - 1), float(Z - 1)));
     r.invVoxel = mk3(1.0f / r.voxel.x, 1.0f / r.voxel.y, 1.0f / r.voxel.z);
-    d3 start, dir;
-    camera_ray(P, x, y, b, start, dir);
     r.start = mk3(float(start.x), float(start.y), float(start.z));
     r.dir = mk3(float(dir.x), float(dir.y), float(dir.z));
-    r.step = P.st ? P.st[b * P.stSb + y * P.stSy + x * P.stSx] : P.stU;
+    r.step = step;
     {
-        const double vx = bsize.x / double(P.X - 1), vy = bsize.y / double(P.Y - 1), vz = bsize.z / double(P.Z - 1);
+        const double vx = bsize.x / double(X - 1), vy = bsize.y / double(Y - 1), vz = bsize.z / double(Z - 1);
         r.p0 = mk3(float((start.x - bmin.x) / vx), float((start.y - bmin.y) / vy), float((start.z - bmin.z) / vz));
         r.dp = mk3(float(dir.x / vx), float(dir.y / vy), float(dir.z / vz));
     }
@@ -190,6 +210,27 @@ __device__ __forceinline__ Ray setup_ray(const RenderParams& P, int x, int y, in
         if (!(r.step > 0.0f) || !isfinite(r.step)) r.step = 1.0f;
     }
     return r;
+}
+
+__device__ __forceinline__ Ray setup_ray(const RenderParams& P, int x, int y, int b)
+{
+    d3 bmin, bsize;
+    if (P.bmin) {
+        const float* p = P.bmin + b * P.bminSb;
+        bmin = dk3(p[0], p[P.bminSc], p[2 * P.bminSc]);
+    } else {
+        bmin = dk3(P.bminU[0], P.bminU[1], P.bminU[2]);
+    }
+    if (P.bsize) {
+        const float* p = P.bsize + b * P.bsizeSb;
+        bsize = dk3(p[0], p[P.bsizeSc], p[2 * P.bsizeSc]);
+    } else {
+        bsize = dk3(P.bsizeU[0], P.bsizeU[1], P.bsizeU[2]);
+    }
+    d3 start, dir;
+    camera_ray(P, x, y, b, start, dir);
+    const float step = P.st ? P.st[b * P.stSb + y * P.stSy + x * P.stSx] : P.stU;
+    return make_ray(start, dir, bmin, bsize, P.X, P.Y, P.Z, step);
 }
 
 // volume-space sample position of parameter t: ((start + t dir) - bmin) / voxel,
@@ -636,8 +677,8 @@ __device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4
 }
 
 struct SinkNone {
-    __device__ void add(int, int, float) {}
-    template <bool POS> __device__ void add_pair(int, float, int, float, const f4&, float, float) {}
+    __device__ __forceinline__ void add(int, int, float) {}
+    template <bool POS> __device__ __forceinline__ void add_pair(int, float, int, float, const f4&, float, float) {}
 };
 
 // Nearest-neighbour voxel indices (renderer_interpolation.cuh:38-50)

@@ -47,23 +47,6 @@ __global__ void k_zero_viewport(float* out, int oSb, int oSr, int oSc, int B)
     out[b * oSb + (k / 3) * oSr + (k % 3) * oSc] = 0.0f;
 }
 
-// adjNormalize (renderer_adjoint.cuh:28-45)
-__device__ __forceinline__ f3 adj_normalize(f3 in, f3 adj_out)
-{
-    const float len = sqrtf(dot3(in, in));
-    f3 adj_in = scl3(adj_out, 1.0f / len);
-    const float adj_len = dot3(adj_out, scl3(in, -1.0f / (len * len)));
-    return add3(adj_in, scl3(in, adj_len / len));
-}
-
-// adjCross (renderer_adjoint.cuh:47-65)
-__device__ __forceinline__ void adj_cross(f3 a, f3 b, f3 c, f3& aa, f3& ab)
-{
-    aa.y += b.z * c.x; ab.z += a.y * c.x; aa.z -= b.y * c.x; ab.y -= a.z * c.x;
-    aa.z += b.x * c.y; ab.x += a.z * c.y; aa.x -= b.z * c.y; ab.z -= a.x * c.y;
-    aa.x += b.y * c.z; ab.y += a.x * c.z; aa.y -= b.x * c.z; ab.x -= a.y * c.z;
-}
-
 __global__ __launch_bounds__(256) void k_generate_rays_adjoint(const float* vp, int vSb, int vSr, int vSc, int H,
                                                                int W, float tanX, float tanY, const float* ars,
                                                                int4 arsS, const float* ard, int4 ardS, float* out,

@@ -53,4 +53,16 @@ hipError_t launch_generate_rays_adjoint(const float* viewport, int vSb, int vSr,
                                         float fovY, const float* ars, const int arsS[4], const float* ard,
                                         const int ardS[4], float* out, int oSb, int oSr, int oSc, hipStream_t s);
 
+// dvr_kernels_opt.hip
+hipError_t launch_zero_segments(const ZeroSeg* dSegs, int nSegs, long long maxN, hipStream_t s);
+// increments *dStep, then updates every block with step count *dStep
+hipError_t launch_adam(const AdamSeg* dSegs, int nSegs, long long maxN, long long* dStep, double lr, double b1,
+                       double b2, double eps, hipStream_t s);
+
+// dvr_kernels_batched.hip: launch the batched persistent kernel (occ == NULL)
+// or report resident workgroups per CU of that instance.  gm: 0 forward only,
+// 1 TF, 3 volume (+TF), 7 volume + TF + camera.
+hipError_t launch_batched(const BatchParams& P, int filter, int tfm, int blend, int K, int gm, int grid, size_t lds,
+                          hipStream_t s, int* occ);
+
 }  // namespace dvr

@@ -54,11 +54,13 @@ struct AdjointParams {
 };
 
 // Checkpoints: ckpt c (0-based) of a ray holds the transmittance 1 - alpha
-// accumulated after (c+1)*K samples.  Layout [tile][c][thread] so one wave stores 256 B.
+// accumulated after (c+1)*K*every samples.  Layout [tile][c][thread] so one
+// wave stores 256 B.  The reverse march walks K-sample register segments.
 struct CkptParams {
     float* buf;   // NULL = none
-    int K;        // checkpoint interval
+    int K;        // register segment length (8, 16, 32)
     int cap;      // checkpoints per ray
+    int every;    // checkpoint interval in segments (1 or 2): interval = K*every
 };
 
 // Per-pixel loss whose gradient the fused forward+adjoint kernel forms in
@@ -72,6 +74,78 @@ struct LossParams {
     const float* target; int tgSb, tgSy, tgSx, tgSc;
     float scale;          // 1 / number of loss elements
     float* lossOut;       // optional scalar accumulator (atomic), may be NULL
+};
+
+
+// ---------------------------------------------------------------------------
+// Batched persistent launch: many views (cameras) of many volumes and TFs in
+// one kernel.  The views are cut into workgroup tiles; tiles are numbered
+// view after view and split into one contiguous range per XCD.  Persistent
+// workgroups pop tiles from their XCD's queue and steal from the others when
+// it runs dry.  Volumes and TFs are contiguous (x slowest, z fastest; row
+// major R x C).
+// ---------------------------------------------------------------------------
+struct BView {
+    int vol, tf, W, H;
+    int tileBegin, tilesX, camMode, pad0;
+    double tanX, tanY;          // ReferenceFrame tangents (host, double)
+    const float* cam;           // 3x3 viewport rows (eye, right, up) or 4x4 row-major matrix
+    float step, lossScale;
+    float* color;               // H x W x 4
+    int* term;                  // H x W
+    const float* target;        // H x W x 4, loss target (loss kinds L1/L2)
+    const float* acol;          // H x W x 4, given adjoint colour (loss kind none)
+    float* acam;                // 3 x 3 viewport adjoint (accumulated) or NULL
+};
+
+struct BVol {
+    const float* p;
+    float* adj;                 // gradient (accumulated) or NULL
+    int X, Y, Z, pad0;
+    float bmin[3], bsize[3];
+};
+
+struct BTf {
+    const float* p;
+    float* adj;                 // gradient (accumulated) or NULL
+    int R, C;
+};
+
+struct BatchParams {
+    const BView* views;
+    const BVol* vols;
+    const BTf* tfs;
+    unsigned* queues;           // kNumXcd tile counters, zeroed before every launch
+    float* ck;                  // checkpoints [workgroup][cap][kBlock]
+    float* lossOut;             // scalar loss accumulator or NULL
+    int nViews, nTiles, ckCap;
+    int ckEvery;                // checkpoint interval = K * ckEvery samples
+    int qBegin[kNumXcd + 1];    // tile range of each XCD's queue
+    float earlyOut;
+    int lossKind;               // DVR_LOSS_*
+    int wantTf, wantVol, wantCam;
+    int waveShift;              // log2 of the wave tile width; wave tile = (1<<s) x (64>>s) rays
+    int tileW, tileH;           // workgroup tile = 2 x 2 wave tiles
+    int ldsTf;                  // floats of LDS per TF table (max R*C over the TFs)
+    int usePriv;                // Gaussian TF gradient in per-thread LDS slices
+};
+
+// One segment of a zero-fill list (gradients, counters) cleared by a kernel
+// (graph-capturable; no memset nodes).
+struct ZeroSeg {
+    float* p;
+    long long n;
+};
+
+// One parameter block of the multi-block Adam step: rows x cols floats with a
+// row stride shared by param, grad and both moment buffers.
+struct AdamSeg {
+    float* p;
+    const float* grad;
+    float* m;
+    float* v;
+    long long rows, cols, stride;
+    float lo, hi;               // clamp after the update when lo <= hi
 };
 
 }  // namespace dvr

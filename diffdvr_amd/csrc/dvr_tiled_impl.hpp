@@ -114,7 +114,7 @@ struct SinkTiled {
     float a0[5], a1[5];
     float id0, id1;
 
-    __device__ void init(float* table, int C_, bool usePriv)
+    __device__ __forceinline__ void init(float* table, int C_, bool usePriv)
     {
         tab = table; C = C_; priv = usePriv;
         r0 = r1 = -1;
@@ -122,7 +122,7 @@ struct SinkTiled {
         for (int k = 0; k < 5; ++k) { a0[k] = 0.f; a1[k] = 0.f; }
         id0 = id1 = 0.f;
     }
-    __device__ void add(int r, int c, float v)
+    __device__ __forceinline__ void add(int r, int c, float v)
     {
         if constexpr (TFM == 0) {
             if (c == 0) id0 += v; else id1 += v;
@@ -131,27 +131,32 @@ struct SinkTiled {
             else atomicAdd(&tab[r * C + c], v);
         }
     }
-    __device__ void flush_row(int r, float a[5], int ncol)
+    template <int NC>
+    __device__ __forceinline__ void flush_row(int r, float (&a)[5])
     {
-        if (r >= 0)
-            for (int c = 0; c < ncol; ++c)
+        if (r >= 0) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
                 if (a[c] != 0.0f) atomicAdd(&tab[r * C + c], a[c]);
+        }
 #pragma unroll
         for (int k = 0; k < 5; ++k) a[k] = 0.f;
     }
     template <bool POS>
-    __device__ void add_pair(int i0, float w0, int i1, float w1, const f4& adj, float p0, float p1)
+    __device__ __forceinline__ void add_pair(int i0, float w0, int i1, float w1, const f4& adj, float p0, float p1)
     {
         constexpr int NC = POS ? 5 : 4;
-        if (i0 != r0) { flush_row(r0, a0, NC); r0 = i0; }
+        if (i0 != r0) { flush_row<NC>(r0, a0); r0 = i0; }
         a0[0] += adj.x * w0; a0[1] += adj.y * w0; a0[2] += adj.z * w0; a0[3] += adj.w * w0;
         if constexpr (POS) a0[4] += p0;
-        if (i1 != r1) { flush_row(r1, a1, NC); r1 = i1; }
+        if (i1 != r1) { flush_row<NC>(r1, a1); r1 = i1; }
         a1[0] += adj.x * w1; a1[1] += adj.y * w1; a1[2] += adj.z * w1; a1[3] += adj.w * w1;
         if constexpr (POS) a1[4] += p1;
     }
-    // called by every thread of the workgroup (uniform control flow)
-    __device__ void finish(int ncol)
+    // called by every thread of the workgroup (uniform control flow); NC
+    // columns per row (5 with the control-point positions of a linear TF)
+    template <int NC>
+    __device__ __forceinline__ void finish()
     {
         if constexpr (TFM == 0) {
             const float s0 = wave_sum(id0), s1 = wave_sum(id1);
@@ -160,20 +165,22 @@ struct SinkTiled {
                 atomicAdd(&tab[1], s1);
             }
         } else if constexpr (TFM == 1 || TFM == 2) {
-            flush_row(r0, a0, ncol);
-            flush_row(r1, a1, ncol);
+            flush_row<NC>(r0, a0);
+            flush_row<NC>(r1, a1);
         }
     }
 };
 
 // ---------------------------------------------------------------------------
-// Reverse march of one ray with alpha checkpoints.  GM is the set of gradient
+// Reverse march of one ray with transmittance checkpoints every K*ckEvery
+// samples, walked back in register segments of K samples.  GM is the set of gradient
 // kinds compiled in: bit0 TF, bit1 volume, bit2 position (camera / step);
 // the runtime flags select among them.
 // ---------------------------------------------------------------------------
 template <int FILTER, int TFM, int BLEND, int K, int GM, class Sink>
 __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const Ray& r, int n, f4 adj_acc,
-                                              const float* ck, int ckS, int cap, Sink& sink, bool wantTf,
+                                              const float* ck, int ckS, int cap, int ckEvery, Sink& sink,
+                                              bool wantTf,
                                               bool wantVol, bool wantPos, float* avol, int avx, int avy, int avz,
                                               f3& adj_start, f3& adj_dir, float& adj_step)
 {
@@ -216,11 +223,13 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
         // transmittance before sample i0
         float Tr = 1.0f;
         {
-            const int c = min(seg, cap);
+            // checkpoints hold T after every K*ckEvery samples; recompute from the
+            // last one before i0 (ckEvery > 1, or a ray longer than the capacity)
+            const int c = min(seg / ckEvery, cap);
             if (c > 0) Tr = ck[(c - 1) * ckS];
             Cell cc;
             cell_invalidate(cc);
-            for (int i = c * K; i < i0; ++i) {  // only when the ray outgrew the checkpoint capacity
+            for (int i = c * K * ckEvery; i < i0; ++i) {
                 const float t = r.t0 + float(i) * r.step;
                 const float d = fetch<FILTER>(V, sample_pos(r, t), cc);
                 float alpha, keep;
@@ -339,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_forward_tiled(RenderParams P, CkptPa
     float* ck = CK.buf ? CK.buf + (size_t(tc.tile) * CK.cap) * kBlock + threadIdx.x : nullptr;
     Acc acc;
     int n;
-    march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, CK.K, CK.cap);
+    march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, CK.K * CK.every, CK.cap);
     float* o = P.color + tc.b * P.colSb + tc.y * P.colSy + tc.x * P.colSx;
     o[0] = acc.r; o[P.colSc] = acc.g; o[2 * P.colSc] = acc.b; o[3 * P.colSc] = acc.A;
     P.term[tc.b * P.termSb + tc.y * P.termSy + tc.x * P.termSx] = n;
@@ -377,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
         f4 adj_acc;
         if (fused) {
             Acc acc;
-            march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, K, CK.cap);
+            march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, K * CK.every, CK.cap);
             float* o = P.color + b * P.colSb + y * P.colSy + x * P.colSx;
             o[0] = acc.r; o[P.colSc] = acc.g; o[2 * P.colSc] = acc.b; o[3 * P.colSc] = acc.A;
             P.term[b * P.termSb + y * P.termSy + x * P.termSx] = n;
@@ -399,14 +408,14 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
             if (!ckFromFwd) {
                 Acc acc;
                 int n2;
-                march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, n, acc, n2, ck, kBlock, K, CK.cap);
+                march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, n, acc, n2, ck, kBlock, K * CK.every, CK.cap);
             }
         }
         f3 adj_start = mk3(0.f, 0.f, 0.f), adj_dir = mk3(0.f, 0.f, 0.f);
         float adj_step = 0.0f;
         const bool wantVol = A.avol != nullptr;
         const bool wantPos = A.ars != nullptr || A.ast != nullptr;
-        march_reverse<FILTER, TFM, BLEND, K, GM>(V, T, r, n, adj_acc, ck, kBlock, CK.cap, sink, wantTf, wantVol,
+        march_reverse<FILTER, TFM, BLEND, K, GM>(V, T, r, n, adj_acc, ck, kBlock, CK.cap, CK.every, sink, wantTf, wantVol,
                                                  wantPos, wantVol ? A.avol + b * A.avSb : nullptr, A.avSx, A.avSy,
                                                  A.avSz, adj_start, adj_dir, adj_step);
         if (A.ast) {
@@ -426,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
             }
         }
     }
-    if (wantTf) sink.finish(TFM == 2 ? 5 : 4);
+    if (wantTf) sink.template finish<TFM == 2 ? 5 : 4>();
     if (fused && L.lossOut) {
         const float s = wave_sum(lossLocal);
         if ((threadIdx.x & 63) == 0) atomicAdd(L.lossOut, s);

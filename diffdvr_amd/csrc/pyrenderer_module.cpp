@@ -271,6 +271,99 @@ struct GpuTimer {
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// batched plans and the Adam group: thin handles over the C-ABI; tensors are
+// validated and kept alive by diffdvr_amd/batch.py, which passes raw pointers.
+// ---------------------------------------------------------------------------
+struct BatchPlanHandle {
+    dvr_batch_plan* plan = nullptr;
+    ~BatchPlanHandle()
+    {
+        if (plan) (void)dvr_batch_plan_destroy(plan);
+    }
+};
+
+template <class T> T* ptr_of(const py::handle& h) { return reinterpret_cast<T*>(h.cast<uintptr_t>()); }
+
+std::unique_ptr<BatchPlanHandle> make_plan(py::list volumes, py::list tfs, py::list views, int filter, int tfm,
+                                           int blend, float early, int loss, uintptr_t loss_out, bool forward_only,
+                                           int device, int K, int ww, int grid)
+{
+    std::vector<dvr_batch_volume> vs;
+    for (auto h : volumes) {
+        py::tuple t = py::reinterpret_borrow<py::tuple>(h);
+        if (t.size() != 7) throw std::invalid_argument("volume entry: (data, X, Y, Z, box_min, box_size, adj)");
+        dvr_batch_volume v;
+        std::memset(&v, 0, sizeof(v));
+        v.data = ptr_of<const float>(t[0]);
+        v.size_x = t[1].cast<int>(); v.size_y = t[2].cast<int>(); v.size_z = t[3].cast<int>();
+        py::sequence bm = py::reinterpret_borrow<py::sequence>(t[4]), bs = py::reinterpret_borrow<py::sequence>(t[5]);
+        for (int k = 0; k < 3; ++k) { v.box_min[k] = bm[k].cast<float>(); v.box_size[k] = bs[k].cast<float>(); }
+        v.adj_data = ptr_of<float>(t[6]);
+        vs.push_back(v);
+    }
+    std::vector<dvr_batch_tf> ts;
+    for (auto h : tfs) {
+        py::tuple t = py::reinterpret_borrow<py::tuple>(h);
+        if (t.size() != 4) throw std::invalid_argument("tf entry: (data, rows, cols, adj)");
+        dvr_batch_tf f;
+        std::memset(&f, 0, sizeof(f));
+        f.data = ptr_of<const float>(t[0]);
+        f.rows = t[1].cast<int>(); f.cols = t[2].cast<int>();
+        f.adj_data = ptr_of<float>(t[3]);
+        ts.push_back(f);
+    }
+    std::vector<dvr_batch_view> ws;
+    for (auto h : views) {
+        py::tuple t = py::reinterpret_borrow<py::tuple>(h);
+        if (t.size() != 14)
+            throw std::invalid_argument("view entry: (volume, tf, W, H, camera_mode, camera, fov_y, step, color, "
+                                        "termination, target, adj_color, loss_scale, adj_camera)");
+        dvr_batch_view w;
+        std::memset(&w, 0, sizeof(w));
+        w.volume = t[0].cast<int>(); w.tf = t[1].cast<int>();
+        w.width = t[2].cast<int>(); w.height = t[3].cast<int>();
+        w.camera_mode = t[4].cast<int>();
+        w.camera = ptr_of<const float>(t[5]);
+        w.fov_y_radians = t[6].cast<float>();
+        w.step_size = t[7].cast<float>();
+        w.color = ptr_of<float>(t[8]);
+        w.termination_index = ptr_of<int32_t>(t[9]);
+        w.target = ptr_of<const float>(t[10]);
+        w.adj_color = ptr_of<const float>(t[11]);
+        w.loss_scale = t[12].cast<float>();
+        w.adj_camera = ptr_of<float>(t[13]);
+        ws.push_back(w);
+    }
+    dvr_batch b;
+    std::memset(&b, 0, sizeof(b));
+    b.num_volumes = (int)vs.size(); b.num_tfs = (int)ts.size(); b.num_views = (int)ws.size();
+    b.volumes = vs.data(); b.tfs = ts.data(); b.views = ws.data();
+    b.volume_filter_mode = filter; b.tf_mode = tfm; b.blend_mode = blend; b.blending_early_out = early;
+    b.loss = loss; b.loss_out = reinterpret_cast<float*>(loss_out); b.forward_only = forward_only ? 1 : 0;
+    b.device = device;
+    dvr_batch_options o{K, ww, grid};
+    std::unique_ptr<BatchPlanHandle> hdl(new BatchPlanHandle());
+    check(dvr_batch_plan_create(&b, &o, &hdl->plan));
+    return hdl;
+}
+
+py::tuple plan_options(const BatchPlanHandle& h)
+{
+    dvr_batch_options o;
+    check(dvr_batch_plan_get_options(h.plan, &o));
+    return py::make_tuple(o.checkpoint_interval, o.wave_width, o.grid);
+}
+
+struct AdamHandle {
+    dvr_adam_group* group = nullptr;
+    ~AdamHandle()
+    {
+        if (group) (void)dvr_adam_group_destroy(group);
+    }
+};
+
 }  // namespace
 
 PYBIND11_MODULE(_pyrenderer, m)
@@ -486,4 +579,57 @@ PYBIND11_MODULE(_pyrenderer, m)
               dvr_tensor v = to_tensor(viewport), s = to_tensor(adj_start), d = to_tensor(adj_dir), o = to_tensor(out);
               check(dvr_generate_rays_adjoint(&v, fovY, W, H, &s, &d, &o, current_stream(v.device)));
           });
+
+    py::class_<BatchPlanHandle>(m, "_BatchPlan")
+        .def(py::init(&make_plan), py::arg("volumes"), py::arg("tfs"), py::arg("views"), py::arg("filter"),
+             py::arg("tf_mode"), py::arg("blend"), py::arg("early_out"), py::arg("loss"), py::arg("loss_out"),
+             py::arg("forward_only"), py::arg("device"), py::arg("checkpoint_interval"), py::arg("wave_width"),
+             py::arg("grid"))
+        .def("run", [](BatchPlanHandle& h, int flags, uintptr_t stream) {
+            check(dvr_batch_plan_run(h.plan, flags, reinterpret_cast<void*>(stream)));
+        })
+        .def("zero_gradients", [](BatchPlanHandle& h, uintptr_t stream) {
+            check(dvr_batch_plan_zero_gradients(h.plan, reinterpret_cast<void*>(stream)));
+        })
+        .def("options", &plan_options)
+        .def("set_options", [](BatchPlanHandle& h, int K, int ww, int grid) {
+            dvr_batch_options o{K, ww, grid};
+            check(dvr_batch_plan_set_options(h.plan, &o));
+        })
+        .def("autotune", [](BatchPlanHandle& h, int iterations, uintptr_t stream) {
+            dvr_batch_options o;
+            check(dvr_batch_plan_autotune(h.plan, iterations, &o, reinterpret_cast<void*>(stream)));
+            return py::make_tuple(o.checkpoint_interval, o.wave_width, o.grid);
+        })
+        .def("workspace_bytes", [](const BatchPlanHandle& h) { return dvr_batch_plan_workspace_bytes(h.plan); });
+
+    py::class_<AdamHandle>(m, "_AdamGroup")
+        .def(py::init([](py::list blocks, int device) {
+                 std::vector<dvr_adam_block> bs;
+                 for (auto b : blocks) {
+                     py::tuple t = py::reinterpret_borrow<py::tuple>(b);
+                     if (t.size() != 9)
+                         throw std::invalid_argument("block: (param, grad, exp_avg, exp_avg_sq, rows, cols, stride, "
+                                                     "clamp_min, clamp_max)");
+                     dvr_adam_block a;
+                     a.param = ptr_of<float>(t[0]); a.grad = ptr_of<const float>(t[1]);
+                     a.exp_avg = ptr_of<float>(t[2]); a.exp_avg_sq = ptr_of<float>(t[3]);
+                     a.rows = t[4].cast<int64_t>(); a.cols = t[5].cast<int64_t>(); a.stride = t[6].cast<int64_t>();
+                     a.clamp_min = t[7].cast<float>(); a.clamp_max = t[8].cast<float>();
+                     bs.push_back(a);
+                 }
+                 std::unique_ptr<AdamHandle> h(new AdamHandle());
+                 check(dvr_adam_group_create(bs.data(), (int)bs.size(), device, &h->group));
+                 return h;
+             }),
+             py::arg("blocks"), py::arg("device"))
+        .def("step", [](AdamHandle& h, double lr, double b1, double b2, double eps, uintptr_t stream) {
+            check(dvr_adam_group_step(h.group, lr, b1, b2, eps, reinterpret_cast<void*>(stream)));
+        })
+        .def("set_step", [](AdamHandle& h, int64_t step) { check(dvr_adam_group_set_step(h.group, step)); })
+        .def("get_step", [](AdamHandle& h) {
+            int64_t s = 0;
+            check(dvr_adam_group_get_step(h.group, &s));
+            return s;
+        });
 }

@@ -209,5 +209,87 @@ def workload_c3(device="cpu", views: int = 64, size: int = 256, res: int = 256) 
                     seeds={"volume": seed, "target_volume": seed + 7}).to(device)
 
 
+@dataclass
+class CameraBatch:
+    """Sphere cameras of uneven image sizes around the origin (Orientation.Yp)."""
+    lat: torch.Tensor                 # N x 1 float64 (pitch)
+    lon: torch.Tensor                 # N x 1 float64 (yaw)
+    radius: torch.Tensor              # N x 1 float64
+    sizes: list                       # [(W, H)] per camera
+
+    def viewports(self, device="cpu", dtype=torch.float32) -> torch.Tensor:
+        from . import pyrenderer as pr
+        vp = pr.Camera.viewport_from_sphere(torch.zeros(self.lat.shape[0], 3, dtype=torch.float64), self.lon,
+                                            self.lat, self.radius, pr.Orientation.Yp)
+        return vp.to(dtype).to(device)
+
+
+@dataclass
+class MultiWorkload:
+    """Several volumes, each with its own TF and cameras of uneven image sizes."""
+    name: str
+    volumes: list                     # [X x Y x Z] float32
+    tfs: list                         # [R x C] float32
+    tf_mode: int
+    cameras: list                     # [CameraBatch] per volume
+    steps: list                       # world step per volume
+    target_volumes: list | None = None
+    target_cameras: list | None = None
+    loss: str | None = None
+    grads: tuple = ()
+    filter: int = 1
+    blend: int = 0
+    fov_y: float = FOV_Y
+    seeds: dict = field(default_factory=dict)
+
+
+def workload_c4(device="cpu", cameras: int = 512) -> MultiWorkload:
+    """C4: 128^3 blobs, 4-component Gaussian TF, `cameras` sphere cameras with seeded
+    latitude / longitude and image sizes from {64,128,256,512}^2; gradient w.r.t. every
+    camera's latitude, longitude and radius (L2 against images from perturbed cameras)."""
+    seed = 4004
+    vol = blob_volume(128, 32, seed, device=device)[0]
+    tf = gaussian_tf(4, seed + 1)[0].to(device)
+    rng = Rng(seed + 2)
+    lat = np.arcsin(rng.uniform(cameras, -0.95, 0.95))
+    lon = rng.uniform(cameras, 0.0, 2.0 * math.pi)
+    side = rng.choice([64, 128, 256, 512], cameras)
+    dlat = rng.uniform(cameras, -0.05, 0.05)
+    dlon = rng.uniform(cameras, -0.05, 0.05)
+    drad = rng.uniform(cameras, -0.1, 0.1)
+    col = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).reshape(-1, 1)
+    sizes = [(int(s), int(s)) for s in side]
+    cams = CameraBatch(col(lat), col(lon), col(np.full(cameras, 2.5)), sizes)
+    target = CameraBatch(col(lat + dlat), col(lon + dlon), col(2.5 + drad), sizes)
+    return MultiWorkload("C4", [vol], [tf], 3, [cams], [0.5 / 128], target_cameras=[target], loss="l2",
+                         grads=("camera",), seeds={"volume": seed, "tf": seed + 1, "cameras": seed + 2})
+
+
+def workload_c5(device="cpu", volumes: int = 16) -> MultiWorkload:
+    """C5: `volumes` blob volumes of seeded sizes {64,96,128,192,256}^3, each with its own
+    5-point linear TF and 4-32 sphere cameras at seeded square resolutions 128..512
+    (multiples of 32), step 0.5 voxel; L2 against images of a second seeded volume
+    through the same TF; gradient w.r.t. density and TF jointly."""
+    seed = 5005
+    rng = Rng(seed)
+    sides = rng.choice([64, 96, 128, 192, 256], volumes)
+    vols, tgts, tfs, cams, steps = [], [], [], [], []
+    for v in range(volumes):
+        n = int(sides[v])
+        vols.append(blob_volume(n, 16, seed + 100 * v + 1, device=device)[0])
+        tgts.append(blob_volume(n, 16, seed + 100 * v + 2, device=device)[0])
+        tfs.append(linear_tf(5, seed + 100 * v + 3)[0].to(device))
+        nc = int(rng.choice(np.arange(4, 33), 1)[0])
+        lat = np.arcsin(rng.uniform(nc, -0.95, 0.95))
+        lon = rng.uniform(nc, 0.0, 2.0 * math.pi)
+        res = rng.choice(np.arange(128, 513, 32), nc)
+        col = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).reshape(-1, 1)
+        cams.append(CameraBatch(col(lat), col(lon), col(np.full(nc, 2.5)), [(int(r), int(r)) for r in res]))
+        steps.append(0.5 / n)
+    return MultiWorkload("C5", vols, tfs, 2, cams, steps, target_volumes=tgts, loss="l2", grads=("volume", "tf"),
+                         seeds={"base": seed})
+
+
 __all__ = ["Rng", "splitmix64", "blob_volume", "linear_tf", "texture_tf", "gaussian_tf", "identity_tf",
-           "fibonacci_sphere", "sphere_viewports", "Workload", "workload_c1", "workload_c2", "workload_c3", "FOV_Y"]
+           "fibonacci_sphere", "sphere_viewports", "Workload", "workload_c1", "workload_c2", "workload_c3", "FOV_Y",
+           "CameraBatch", "MultiWorkload", "workload_c4", "workload_c5"]

@@ -120,7 +120,7 @@ typedef struct dvr_forward_options {
 typedef struct dvr_adjoint_options {
     int32_t path;                /* DVR_PATH_* */
     int32_t method;              /* DVR_ADJOINT_* */
-    int32_t checkpoint_interval; /* K in {8,16,32}; 0 = autotuned/default */
+    int32_t checkpoint_interval; /* K in {8,16,32,64}; 0 = default (16) */
     dvr_tensor checkpoints;      /* optional: written by dvr_render_forward_ex with the same K */
 } dvr_adjoint_options;
 
@@ -154,7 +154,7 @@ dvr_status dvr_render_adjoint_ex(const dvr_inputs* inputs, const dvr_outputs* ou
                                  const dvr_tensor* adj_color, dvr_adjoint_outputs* adj_outputs,
                                  const dvr_adjoint_options* options, void* stream);
 
-/* Per-pixel image loss whose gradient dvr_render_fused forms in registers. */
+/* Per-pixel image loss whose gradient the fused kernels form in registers. */
 #define DVR_LOSS_L1 1 /* mean |color - target|   (torch.nn.L1Loss)  */
 #define DVR_LOSS_L2 2 /* mean (color - target)^2 (torch.nn.MSELoss) */
 typedef struct dvr_loss {
@@ -167,7 +167,7 @@ typedef struct dvr_loss {
 /* Forward march, image loss against `target` and checkpointed adjoint in ONE
  * launch per batch: colour and termination index are written to `outputs`,
  * the loss gradient never leaves registers, and the adjoint outputs are
- * accumulated as by dvr_render_adjoint.  checkpoint_interval K in {0,8,16,32}
+ * accumulated as by dvr_render_adjoint.  checkpoint_interval K in {0,8,16,32,64}
  * (0 = default).  No reference counterpart: it fuses Renderer::renderForward,
  * the caller's torch loss (pytests/tests/volume/compare_reconstruction.py:359-366)
  * and Renderer::renderAdjoint. */
@@ -190,6 +190,126 @@ dvr_status dvr_generate_rays(const dvr_tensor* viewport, float fov_y_radians, in
 dvr_status dvr_generate_rays_adjoint(const dvr_tensor* viewport, float fov_y_radians, int32_t width,
                                      int32_t height, const dvr_tensor* adj_ray_start,
                                      const dvr_tensor* adj_ray_dir, dvr_tensor* adj_viewport, void* stream);
+
+/* ---- batched persistent rendering ---------------------------------------
+ *
+ * Many views (cameras) of many volumes and transfer functions, of uneven image
+ * size and resolution, rendered forward, compared with a target image and
+ * differentiated in ONE persistent launch.  The reference renders a batch
+ * only when all views share one image size and volume shape
+ * (renderer/renderer.cpp:48-391): a caller with uneven views loops over
+ * Renderer::renderForward / renderAdjoint per view
+ * (pytests/diffdvr/renderer.py:127-213) and sums the gradients in torch.
+ *
+ * A plan binds device buffers once (set-up: it allocates and uploads the
+ * descriptor tables and checkpoint scratch); dvr_batch_plan_run then only
+ * launches kernels on the given stream -- no allocation, no host copy, no
+ * memset node -- so a caller may capture optimisation steps into a HIP graph.
+ * The plan keeps POINTERS: volume, TF and camera data are read at every run,
+ * so in-place parameter updates are seen by the next run.  All buffers must
+ * outlive the plan. */
+
+typedef struct dvr_batch_volume {
+    const float* data;       /* X x Y x Z contiguous (z fastest), device memory */
+    int32_t size_x, size_y, size_z;
+    float box_min[3], box_size[3];
+    float* adj_data;         /* X x Y x Z gradient, accumulated; NULL = none */
+} dvr_batch_volume;
+
+typedef struct dvr_batch_tf {
+    const float* data;       /* R x C contiguous, device memory */
+    int32_t rows, cols;
+    float* adj_data;         /* R x C gradient, accumulated; NULL = none */
+} dvr_batch_tf;
+
+typedef struct dvr_batch_view {
+    int32_t volume, tf;      /* indices into the volume / TF tables */
+    int32_t width, height;
+    int32_t camera_mode;     /* DVR_CAMERA_REFERENCE_FRAME or DVR_CAMERA_INVERSE_VIEW_MATRIX */
+    const float* camera;     /* device: 3x3 viewport rows (eye, right, up) or 4x4 row-major matrix */
+    float fov_y_radians;     /* ReferenceFrame */
+    float step_size;
+    float* color;            /* H x W x 4 */
+    int32_t* termination_index; /* H x W */
+    const float* target;     /* H x W x 4 loss target (loss kind L1/L2) */
+    const float* adj_color;  /* H x W x 4 given adjoint of the colour (loss kind none) */
+    float loss_scale;        /* factor of this view's summed loss; 0 = 1/(4 * all pixels of the batch) */
+    float* adj_camera;       /* 3 x 3 viewport gradient (ReferenceFrame), accumulated; NULL = none */
+} dvr_batch_view;
+
+#define DVR_LOSS_NONE 0      /* adjoint colour given per view */
+
+typedef struct dvr_batch {
+    int32_t num_volumes, num_tfs, num_views;
+    const dvr_batch_volume* volumes;  /* host arrays, copied at plan creation */
+    const dvr_batch_tf* tfs;
+    const dvr_batch_view* views;
+    int32_t volume_filter_mode, tf_mode, blend_mode;
+    float blending_early_out;         /* 0 = 1 - 1e-5 */
+    int32_t loss;                     /* DVR_LOSS_NONE / L1 / L2 */
+    float* loss_out;                  /* device scalar, the loss is ADDED to it; NULL = none */
+    int32_t forward_only;             /* 1 = colour and termination only */
+    int32_t device;                   /* HIP device ordinal of every buffer */
+} dvr_batch;
+
+typedef struct dvr_batch_options {
+    int32_t checkpoint_interval; /* K in {8,16,32,64}; 0 = 16 */
+    int32_t wave_width;          /* rays along x of a wave's tile: 4, 8, 16 or 32 (tile 4x16 .. 32x2); 0 = 8 */
+    int32_t grid;                /* persistent workgroups; 0 = resident capacity of the device */
+} dvr_batch_options;
+
+typedef struct dvr_batch_plan dvr_batch_plan;
+
+#define DVR_BATCH_ZERO_GRADIENTS 1   /* run: clear every gradient buffer and loss_out first */
+
+dvr_status dvr_batch_plan_create(const dvr_batch* batch, const dvr_batch_options* options, dvr_batch_plan** plan);
+dvr_status dvr_batch_plan_destroy(dvr_batch_plan* plan);
+/* One persistent launch (plus one clearing kernel) on `stream`. */
+dvr_status dvr_batch_plan_run(dvr_batch_plan* plan, int32_t flags, void* stream);
+/* Clear the gradient buffers and loss_out with a kernel (no memset). */
+dvr_status dvr_batch_plan_zero_gradients(dvr_batch_plan* plan, void* stream);
+dvr_status dvr_batch_plan_get_options(const dvr_batch_plan* plan, dvr_batch_options* options);
+/* Re-layout for new options; may allocate (set-up only, not under capture). */
+dvr_status dvr_batch_plan_set_options(dvr_batch_plan* plan, const dvr_batch_options* options);
+/* Autotuner: times every (K, wave_width) candidate on this plan's own batch
+ * (`iterations` timed runs each, HIP events), keeps the fastest and writes it
+ * to *chosen.  Results are cached per device and batch signature for the
+ * process.  Leaves outputs overwritten and gradients ZEROED. */
+dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr_batch_options* chosen, void* stream);
+/* Bytes of device scratch the plan holds (checkpoints, tables). */
+int64_t dvr_batch_plan_workspace_bytes(const dvr_batch_plan* plan);
+
+/* ---- optimiser step ------------------------------------------------------
+ * Adam (torch.optim.Adam, no weight decay, no amsgrad; the reference's
+ * optimisation scripts use it, pytests/tests/volume/compare_reconstruction.py:64,414)
+ * over several parameter blocks in one launch.  Block b is rows x cols floats
+ * with row stride `stride` shared by param, grad, exp_avg and exp_avg_sq;
+ * after the update the parameter is clamped to [clamp_min, clamp_max] when
+ * clamp_min <= clamp_max. */
+typedef struct dvr_adam_block {
+    float* param;
+    const float* grad;
+    float* exp_avg;
+    float* exp_avg_sq;
+    int64_t rows, cols, stride;
+    float clamp_min, clamp_max;
+} dvr_adam_block;
+
+typedef struct dvr_adam_group dvr_adam_group;
+/* Uploads the block table once (set-up). */
+dvr_status dvr_adam_group_create(const dvr_adam_block* blocks, int32_t num_blocks, int32_t device,
+                                 dvr_adam_group** group);
+dvr_status dvr_adam_group_destroy(dvr_adam_group* group);
+/* One Adam step: advances the group's device-side step counter t (0 at
+ * creation) and updates every block with bias corrections 1 - beta^t.  Two
+ * launches, no allocation, no host copy: captured steps keep counting across
+ * graph replays.  lr, betas and eps are given in double as torch's Python
+ * floats and rounded to float where torch rounds them. */
+dvr_status dvr_adam_group_step(dvr_adam_group* group, double lr, double beta1, double beta2, double eps,
+                               void* stream);
+/* Set-up only (synchronous copies): set or read the step counter. */
+dvr_status dvr_adam_group_set_step(dvr_adam_group* group, int64_t step);
+dvr_status dvr_adam_group_get_step(dvr_adam_group* group, int64_t* step);
 
 /* ---- GPU timer (bindings/bindings.cpp:99-129) ---------------------------- */
 typedef struct dvr_timer dvr_timer;

@@ -724,4 +724,79 @@ inline void render_adjoint(const Inputs<T>& in, const T* color, const int* term,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Kink census (test support).  Counts the samples of a ray (sample count n
+// frozen) whose FP64 density lies within eps * (largest corner magnitude of
+// its cell, the scale of FP32 interpolation error) of a point where the
+// reference's per-sample derivative jumps by a finite amount:
+//   identity TF: the clamp mask at density 0 and 1 (renderer_tf.cuh:62-73,103);
+//   texture TF: texel edges, density*R - 0.5 integer (renderer_tf.cuh:160-171);
+//   linear TF: control points (renderer_tf.cuh:294-313);
+//   alpha blending: the clamp at absorption*step = 1 (renderer_blending.cuh:118-125);
+//   nearest filtering: the rounding edges p = k + 0.5, where the density itself
+//     jumps (renderer_interpolation.cuh:38-50); counted within eps_pos voxels.
+// An FP32 evaluation may place such a sample on the other side of the kink
+// than FP64 does; its adjoint then differs by the jump, which no FP32
+// implementation of the reference's mathematics avoids.  (Trilinear cell
+// faces are kinks too, but the jump of the position derivative there is a
+// second difference of the volume, small for smooth volumes.)
+// ---------------------------------------------------------------------------
+template <class T>
+inline int kink_samples(const Inputs<T>& in, int x, int y, int b, int n, T eps, T eps_pos)
+{
+    const Ray<T> r = setup_ray(in, x, y, b);
+    const T* tf = in.tf + (size_t)bidx(in.tf_b, b) * in.R * in.C;
+    const int vb = bidx(in.vol_b, b);
+    int count = 0;
+    for (int i = 0; i < n; ++i) {
+        const T t = r.t0 + T(i) * r.step;
+        const V3<T> p = sample_pos(r, t);
+        // density and the magnitude its FP32 rounding error scales with
+        T d, scale;
+        if (in.filter == FilterNearest) {
+            d = fetch_density(in, b, p);
+            scale = std::fabs(d);
+        } else {
+            TriCell<T> c;
+            trilinear_cell(in, vb, p, c);
+            d = trilinear_value(c);
+            scale = T(0);
+            for (int k = 0; k < 8; ++k) scale = std::max(scale, std::fabs(c.d[k]));
+        }
+        const T band = eps * scale;
+        bool kink = false;
+        if (in.tf_mode == TFIdentity) {
+            kink = std::fabs(d) < band || std::fabs(d - T(1)) < band;
+        } else if (in.tf_mode == TFTexture) {
+            const T u = d * T(in.R) - T(0.5);
+            kink = std::fabs(u - std::round(u)) < band * T(in.R);
+        } else if (in.tf_mode == TFLinear) {
+            for (int k = 0; k < in.R; ++k) kink |= std::fabs(d - tf[k * in.C + 4]) < band;
+        }
+        if (in.blend == BlendAlpha) {
+            const T a = tf_eval(in, b, d).w * r.step;
+            kink |= std::fabs(a - T(1)) < eps;
+        }
+        if (in.filter == FilterNearest) {
+            const T q[3] = {p.x, p.y, p.z};
+            for (int k = 0; k < 3; ++k) kink |= std::fabs(q[k] - std::floor(q[k]) - T(0.5)) < eps_pos;
+        }
+        count += kink ? 1 : 0;
+    }
+    return count;
+}
+
+template <class T>
+inline void kink_census(const Inputs<T>& in, const int* term, T eps, T eps_pos, int* out)
+{
+    const long long count = (long long)in.B * in.H * in.W;
+#pragma omp parallel for schedule(guided)
+    for (long long i = 0; i < count; ++i) {
+        const int b = (int)(i / ((long long)in.W * in.H));
+        const int y = (int)((i - (long long)b * in.W * in.H) / in.W);
+        const int x = (int)(i - (long long)in.W * (y + (long long)in.H * b));
+        out[i] = kink_samples(in, x, y, b, term[i], eps, eps_pos);
+    }
+}
+
 }  // namespace oracle

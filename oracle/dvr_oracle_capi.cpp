@@ -78,6 +78,11 @@ template <class T> oracle::Inputs<T> tf_only(int mode, const T* tf, int R, int C
     {                                                                                                          \
         oracle::render_forward<T>(convert<T>(in), color, term, frozen_term, frozen_t0, t0_out);                \
     }                                                                                                          \
+    extern "C" void oracle_kink_census_##SUF(const oracle_inputs* in, const int* term, T eps, T eps_pos,        \
+                                             int* out)                                                         \
+    {                                                                                                          \
+        oracle::kink_census<T>(convert<T>(in), term, eps, eps_pos, out);                                       \
+    }                                                                                                          \
     extern "C" void oracle_render_adjoint_##SUF(const oracle_inputs* in, const T* color, const int* term,       \
                                                   const T* adj_color, const oracle_adjoint_outputs* out,          \
                                                   int method, const T* frozen_t0)                                 \

@@ -185,6 +185,19 @@ def adjoint(scene, color, term, adj_color, grads=("tf",), dtype=np.float64, meth
     return out
 
 
+def kink_census(scene, term, eps=1e-5, eps_pos=2e-5):
+    """Per-ray count (B,H,W) of samples near a kink of the reference's mathematics,
+    evaluated in FP64 with the given sample counts: within eps * (cell magnitude)
+    of a TF clamp / texel edge / control point or of the alpha clamp, or within
+    eps_pos voxels of a nearest-neighbour rounding edge."""
+    p = _Prepared(scene, np.float64)
+    t = np.ascontiguousarray(term, dtype=np.int32)
+    out = np.zeros((p.B, p.H, p.W), dtype=np.int32)
+    lib().oracle_kink_census_f64(ctypes.byref(p.s), _ptr(t), ctypes.c_double(eps), ctypes.c_double(eps_pos),
+                                 _ptr(out))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # components
 # ---------------------------------------------------------------------------

@@ -137,3 +137,43 @@ def test_output_shape_mismatch_and_null_arguments():
     assert b"color must have 8 entries at dimension 2" in l.dvr_last_error()
     assert l.dvr_render_forward(None, ctypes.byref(o), None) == 1
     assert l.dvr_checkpoint_buffer_elements(None, 16) == -1
+
+
+# ---------------------------------------------------------------------------
+# batched plans: validation happens before any HIP call
+# ---------------------------------------------------------------------------
+def _plan(volumes=None, tfs=None, views=None, tf_mode=1, K=16, ww=8, loss=0, forward_only=False, device=0):
+    from diffdvr_amd._native import native as n
+    volumes = volumes if volumes is not None else [(0x1000, 5, 5, 5, [-0.5] * 3, [1.0] * 3, 0x2000)]
+    tfs = tfs if tfs is not None else [(0x3000, 4, 4, 0x4000)]
+    views = views if views is not None else [(0, 0, 8, 6, 2, 0x5000, 0.7, 0.01, 0x6000, 0x7000, 0, 0x8000, 0.0, 0)]
+    return n._BatchPlan(volumes, tfs, views, 1, tf_mode, 0, 1 - 1e-5, loss, 0, forward_only, device, K, ww, 0)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(tfs=[(0x3000, 4, 5, 0)]), "tf 0 must have 4 entries at dimension 2"),
+    (dict(views=[(3, 0, 8, 6, 2, 0x5000, 0.7, 0.01, 0x6000, 0x7000, 0, 0x8000, 0.0, 0)]), "volume index out of range"),
+    (dict(views=[(0, 0, 8, 6, 0, 0x5000, 0.7, 0.01, 0x6000, 0x7000, 0, 0x8000, 0.0, 0)]),
+     "ReferenceFrame or InverseViewMatrix"),
+    (dict(views=[(0, 0, 8, 6, 2, 0x5000, 0.7, -1.0, 0x6000, 0x7000, 0, 0x8000, 0.0, 0)]), "step size must be positive"),
+    (dict(views=[(0, 0, 8, 6, 2, 0x5000, 0.7, 0.01, 0x6000, 0x7000, 0, 0, 0.0, 0)]), "adj_color is NULL"),
+    (dict(loss=1), "target is NULL"),
+    (dict(volumes=[(0x1000, 1, 5, 5, [-0.5] * 3, [1.0] * 3, 0)]), "at least 2 voxels"),
+    (dict(K=12), "checkpoint_interval must be 8, 16, 32 or 64"),
+    (dict(ww=6), "wave_width must be 4, 8, 16 or 32"),
+    (dict(tf_mode=6), "tf mode 6 is not supported"),
+    (dict(device=-1), "GPU only"),
+    (dict(views=[(0, 0, 8, 6, 1, 0x5000, 0.7, 0.01, 0x6000, 0x7000, 0, 0x8000, 0.0, 0x9000)]),
+     "camera gradients need a ReferenceFrame camera"),
+])
+def test_batch_plan_validation(kw, msg):
+    with pytest.raises(RuntimeError, match=msg):
+        _plan(**kw)
+
+
+def test_adam_group_validation():
+    from diffdvr_amd._native import native as n
+    with pytest.raises(RuntimeError, match="NULL buffer"):
+        n._AdamGroup([(0x1000, 0, 0x2000, 0x3000, 1, 4, 4, 1.0, 0.0)], 0)
+    with pytest.raises(RuntimeError, match="invalid shape"):
+        n._AdamGroup([(0x1000, 0x1100, 0x2000, 0x3000, 2, 5, 4, 1.0, 0.0)], 0)

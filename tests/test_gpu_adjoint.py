@@ -73,7 +73,7 @@ def compare(got, want, tol, label="", interior=False):
 
 
 ADJ_PATHS = [("inversion", "plain", 0), ("checkpoint", "tiled", 8), ("checkpoint", "tiled", 16),
-             ("checkpoint", "tiled", 32)]
+             ("checkpoint", "tiled", 32), ("checkpoint", "tiled", 64)]
 
 
 @pytest.mark.parametrize("filter", [0, 1])
@@ -101,7 +101,7 @@ def test_adjoint_matrix(filter, tf_mode, blend):
         print(method, path, K, f"peak opacity {peak:.4f}", {k: f"{v:.1e}" for k, v in errs.items()})
 
 
-@pytest.mark.parametrize("K", [8, 16, 32])
+@pytest.mark.parametrize("K", [8, 16, 32, 64])
 def test_c2_fused_l1_tf_gradient(K):
     wl = scenes.workload_c2(views=2, size=64)
     s = scene_from_workload(wl)

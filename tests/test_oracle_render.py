@@ -234,3 +234,31 @@ def test_generate_rays_closed_form_and_fd():
         lp = sum((x * y).sum() for x, y in zip(oracle.generate_rays(p, H, W, fov), (a_s, a_d)))
         lm = sum((x * y).sum() for x, y in zip(oracle.generate_rays(m, H, W, fov), (a_s, a_d)))
         assert g[idx] == pytest.approx((lp - lm) / (2 * eps), rel=1e-6, abs=1e-8)
+
+
+def test_kink_census():
+    """The census flags exactly the samples that sit on a kink of the reference's mathematics."""
+    # constant density 0.6 along +x, 11 samples per ray (t = 1.0, 1.1, ..., 2.0)
+    s = axis_scene(0.6, 1.7, 0.9, 0.1)
+    _, t = oracle.forward(s)
+    assert np.all(oracle.kink_census(s, t) == 0)        # identity TF, density inside (0, 1)
+    s1 = axis_scene(1.0, 1.7, 0.9, 0.1)
+    assert np.all(oracle.kink_census(s1, t) == t)       # density on the identity clamp at 1
+    # linear TF with a control point exactly at the density: every sample is on the kink
+    s2 = dict(s, tf_mode=2, tf=np.array([[[0.1, 0.2, 0.3, 1.0, 0.0], [0.4, 0.5, 0.6, 2.0, 0.6],
+                                           [0.7, 0.8, 0.9, 3.0, 1.0]]]))
+    assert np.all(oracle.kink_census(s2, t) == t)
+    s2["tf"][0, 1, 4] = 0.61                            # 0.01 away: no kink
+    assert np.all(oracle.kink_census(s2, t) == 0)
+    # texture TF with R = 5: texel edges where density * 5 - 0.5 is an integer, e.g. 0.5
+    s3 = dict(axis_scene(0.5, 1.7, 0.9, 0.1), tf_mode=1, tf=np.ones((1, 5, 4)))
+    assert np.all(oracle.kink_census(s3, t) == t)
+    # alpha blending's clamp at absorption * step = 1: 0.6 * a * 0.1 = 1
+    s4 = axis_scene(0.6, 1.0 / 0.06, 0.9, 0.1, blend=1)
+    _, t4 = oracle.forward(s4, frozen_term=t)
+    assert np.all(oracle.kink_census(s4, t) == t)
+    # nearest filtering, X = 6 (voxel 0.2): world x = 0, 0.1, ..., 1 gives p.x = 0, 0.5, ..., 5, so
+    # the samples at p.x = 0.5, 1.5, 2.5, 3.5, 4.5 lie on rounding edges; p.y = 0.67, 2, 3.33 and
+    # p.z = 0.75, 2.25, 3.75, 5.25 never do
+    s5 = dict(s, filter=0)
+    assert np.all(oracle.kink_census(s5, t) == 5)
