@@ -227,8 +227,8 @@ dvr_batch_plan* create(const dvr_batch* b, const dvr_batch_options* o)
         if (v.size_x < 2 || v.size_y < 2 || v.size_z < 2)
             fail(DVR_ERR_INVALID_ARGUMENT, "volume %d must have at least 2 voxels per dimension, got %dx%dx%d", i,
                  v.size_x, v.size_y, v.size_z);
-        if ((long long)v.size_x * v.size_y * v.size_z >= (1LL << 31))
-            fail(DVR_ERR_INVALID_ARGUMENT, "volume %d is too large for 32-bit element offsets", i);
+        if ((long long)v.size_x * v.size_y * v.size_z > (1LL << 30))
+            fail(DVR_ERR_UNSUPPORTED, "volume %d: one volume may hold at most 2^30 voxels", i);
         for (int k = 0; k < 3; ++k)
             if (!(v.box_size[k] > 0.0f) || !std::isfinite(v.box_size[k]) || !std::isfinite(v.box_min[k]))
                 fail(DVR_ERR_INVALID_ARGUMENT, "volume %d: box size must be positive and finite", i);
@@ -262,7 +262,7 @@ dvr_batch_plan* create(const dvr_batch* b, const dvr_batch_options* o)
         if (v.volume < 0 || v.volume >= b->num_volumes) fail(DVR_ERR_INVALID_ARGUMENT, "view %d: volume index out of range", i);
         if (v.tf < 0 || v.tf >= b->num_tfs) fail(DVR_ERR_INVALID_ARGUMENT, "view %d: tf index out of range", i);
         if (v.width <= 0 || v.height <= 0) fail(DVR_ERR_INVALID_ARGUMENT, "view %d: image size must be positive", i);
-        if ((long long)v.width * v.height * 4 >= (1LL << 31)) fail(DVR_ERR_INVALID_ARGUMENT, "view %d: image too large", i);
+        if ((long long)v.width * v.height * 4 > (1LL << 30)) fail(DVR_ERR_INVALID_ARGUMENT, "view %d: image too large", i);
         if (!v.camera || !v.color || !v.termination_index)
             fail(DVR_ERR_INVALID_ARGUMENT, "view %d: camera, color and termination_index must be given", i);
         if (v.camera_mode != DVR_CAMERA_REFERENCE_FRAME && v.camera_mode != DVR_CAMERA_INVERSE_VIEW_MATRIX)

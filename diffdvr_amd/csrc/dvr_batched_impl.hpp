@@ -35,7 +35,7 @@ __device__ __forceinline__ void batched_flush_tf(const BatchParams& B, int tfi, 
         } else {
             v = s_g[e];
         }
-        if (v != 0.0f) atomicAdd(g + e, v);
+        if (v != 0.0f) gatomic_add(g, unsigned(e), v);
     }
 }
 
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
     float* ck = ADJ ? B.ck + size_t(blockIdx.x) * size_t(B.ckCap) * kBlock + threadIdx.x : nullptr;
     const bool priv = B.usePriv != 0;
     int qoff = 0;
-    int curTf = -1, curRC = 0, curC = 0;
+    int curTf = -1, curRC = 0;
     bool curWantTf = false;
     float lossLocal = 0.0f;
 
@@ -97,10 +97,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
             __syncthreads();
             const BTf tf = B.tfs[tfi];
             curTf = tfi;
-            curC = tf.C;
             curRC = tf.R * tf.C;
             curWantTf = ADJ && B.wantTf && tf.adj != nullptr;
-            for (int e = threadIdx.x; e < curRC; e += kBlock) s_tf[e] = tf.p[e];
+            for (int e = threadIdx.x; e < curRC; e += kBlock) s_tf[e] = gld(tf.p, unsigned(e));
             if (curWantTf) {
                 const int ng = priv ? curRC * kBlock : curRC;
                 for (int e = threadIdx.x; e < ng; e += kBlock) s_g[e] = 0.0f;
@@ -112,9 +111,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
         const int ty = tl / V.tilesX, tx = tl - ty * V.tilesX;
         const int x = tx * B.tileW + lx, y = ty * B.tileH + ly;
         const bool valid = x < V.W && y < V.H;
-        const Tf T{s_tf, curRC / curC, curC, curC, 1};
+        constexpr int kC = tf_cols<TFM>();  // validated: every TF of the plan has this many columns
+        const Tf T{s_tf, curRC / kC, kC, kC, 1};
         SinkTiled<TFM> sink;
-        sink.init(s_g, curC, priv);
+        sink.init(s_g, kC, priv);
         const bool wantCam = ADJ && (GM & 4) && B.wantCam && V.acam != nullptr;
         float cv[9];
 #pragma unroll
@@ -125,12 +125,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
             if (V.camMode == 1) {
                 double m[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) m[k] = V.cam[k];
+                for (int k = 0; k < 16; ++k) m[k] = gld(V.cam, unsigned(k));
                 camera_inverse_view(m, x, y, V.W, V.H, start, dir);
             } else {
-                eye = dk3(V.cam[0], V.cam[1], V.cam[2]);
-                right = dk3(V.cam[3], V.cam[4], V.cam[5]);
-                up = dk3(V.cam[6], V.cam[7], V.cam[8]);
+                eye = dk3(gld(V.cam, 0), gld(V.cam, 1), gld(V.cam, 2));
+                right = dk3(gld(V.cam, 3), gld(V.cam, 4), gld(V.cam, 5));
+                up = dk3(gld(V.cam, 6), gld(V.cam, 7), gld(V.cam, 8));
                 camera_reference_frame(eye, right, up, V.tanX, V.tanY, x, y, V.W, V.H, start, dir);
             }
             const Ray r = make_ray(start, dir, dk3(vol.bmin[0], vol.bmin[1], vol.bmin[2]),
@@ -140,18 +140,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
             int n;
             march_forward<FILTER, TFM, BLEND>(VV, T, r, B.earlyOut, -1, acc, n, ck, kBlock, K * B.ckEvery,
                                               ADJ ? B.ckCap : 0);
-            const int pix = y * V.W + x;
-            float* o = V.color + 4 * pix;
-            o[0] = acc.r; o[1] = acc.g; o[2] = acc.b; o[3] = acc.A;
-            V.term[pix] = n;
+            const unsigned pix = unsigned(y * V.W + x);
+            gst(V.color, 4 * pix, acc.r);
+            gst(V.color, 4 * pix + 1, acc.g);
+            gst(V.color, 4 * pix + 2, acc.b);
+            gst(V.color, 4 * pix + 3, acc.A);
+            gsti(V.term, pix, n);
             if constexpr (ADJ) {
                 f4 adj_acc;
                 if (B.lossKind == DVR_LOSS_NONE) {
-                    const float* ac = V.acol + 4 * pix;
-                    adj_acc = mk4(ac[0], ac[1], ac[2], ac[3]);
+                    adj_acc = mk4(gld(V.acol, 4 * pix), gld(V.acol, 4 * pix + 1), gld(V.acol, 4 * pix + 2),
+                                  gld(V.acol, 4 * pix + 3));
                 } else {
-                    const float* tg = V.target + 4 * pix;
-                    const float d0 = acc.r - tg[0], d1 = acc.g - tg[1], d2 = acc.b - tg[2], d3_ = acc.A - tg[3];
+                    const float d0 = acc.r - gld(V.target, 4 * pix), d1 = acc.g - gld(V.target, 4 * pix + 1),
+                                d2 = acc.b - gld(V.target, 4 * pix + 2), d3_ = acc.A - gld(V.target, 4 * pix + 3);
                     const float sc = V.lossScale;
                     if (B.lossKind == DVR_LOSS_L1) {
                         auto sgn = [](float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); };
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
 #pragma unroll
                     for (int k = 0; k < 9; ++k) {
                         const float s = wave_sum(cv[k]);
-                        if (lane == 0 && s != 0.0f) atomicAdd(V.acam + k, s);
+                        if (lane == 0 && s != 0.0f) gatomic_add(V.acam, unsigned(k), s);
                     }
                 }
             }

@@ -93,6 +93,16 @@ void check_offsets(const dvr_tensor& t, const char* name)
     if (maxo >= (int64_t(1) << 31)) fail(DVR_ERR_INVALID_ARGUMENT, "%s is too large for 32-bit element offsets", name);
 }
 
+// One batch slice of a volume (dims 1..3) is addressed with 32-bit byte
+// offsets by the kernels: its largest element offset must stay below 2^30.
+void check_slice_offsets(const dvr_tensor& t, const char* name)
+{
+    int64_t maxo = 0;
+    for (int d = 1; d < t.ndim; ++d)
+        if (t.sizes[d] > 0) maxo += (t.sizes[d] - 1) * t.strides[d];
+    if (maxo >= (int64_t(1) << 30)) fail(DVR_ERR_UNSUPPORTED, "%s: one volume may hold at most 2^30 voxels", name);
+}
+
 struct Checked {
     dvr::RenderParams P;
     int B, W, H, X, Y, Z;
@@ -125,6 +135,7 @@ Checked check_inputs(const dvr_inputs* in)
     c.Y = (int)vol.sizes[2];
     c.Z = (int)vol.sizes[3];
     check_offsets(vol, "volume");
+    check_slice_offsets(vol, "volume");
     P.vol = (const float*)vol.data;
     P.X = c.X; P.Y = c.Y; P.Z = c.Z;
     P.vsb = bstride(vol, 0); P.vsx = bstride(vol, 1); P.vsy = bstride(vol, 2); P.vsz = bstride(vol, 3);
@@ -308,6 +319,7 @@ dvr::AdjointParams check_adjoint(const Checked& c, const dvr_outputs* fwd, const
         check_batch(t, "adj_volume", c.B);
         check_size(t, "adj_volume", 1, c.X); check_size(t, "adj_volume", 2, c.Y); check_size(t, "adj_volume", 3, c.Z);
         check_dtype(t, "adj_volume", DVR_FLOAT32); check_offsets(t, "adj_volume");
+        check_slice_offsets(t, "adj_volume");
         A.avol = (float*)t.data;
         A.avSb = bstride(t, 0); A.avSx = bstride(t, 1); A.avSy = bstride(t, 2); A.avSz = bstride(t, 3);
     }

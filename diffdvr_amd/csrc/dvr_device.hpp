@@ -57,6 +57,30 @@ __device__ __forceinline__ void adj_cross(f3 a, f3 b, f3 c, f3& aa, f3& ab)
 }
 
 // ---------------------------------------------------------------------------
+// Global memory through a uniform 64-bit base and a 32-bit byte offset per
+// lane (offsets < 2^32 bytes: slices of at most 2^30 floats).  The compiler
+// then emits global_load / global_store / global_atomic in their SGPR-base
+// form instead of flat instructions: a flat access, which it must use for a
+// pointer it cannot prove global (one read from a table), also counts on the
+// LDS wait counter, so every wait for an LDS read would wait for the voxel
+// gathers in flight as well.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) int gint;
+typedef __attribute__((address_space(1))) char gchar;
+
+__device__ __forceinline__ float gld(const float* base, unsigned idx)
+{
+    return *(const gfloat*)((const gchar*)base + idx * 4u);
+}
+__device__ __forceinline__ void gst(float* base, unsigned idx, float v) { *(gfloat*)((gchar*)base + idx * 4u) = v; }
+__device__ __forceinline__ void gsti(int* base, unsigned idx, int v) { *(gint*)((gchar*)base + idx * 4u) = v; }
+__device__ __forceinline__ void gatomic_add(float* base, unsigned idx, float v)
+{
+    __hip_atomic_fetch_add((gfloat*)((gchar*)base + idx * 4u), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
 // Wave-level helpers (wave64)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float wave_sum(float v)
@@ -263,20 +287,38 @@ struct Cell {
 
 __device__ __forceinline__ void cell_invalidate(Cell& c) { c.ix = c.iy = c.iz = -0x40000000; }
 
+// Corner offsets of cell (ix, iy, iz) with the reference's per-corner clamp
+// (renderer_interpolation.cuh:150-157): the low corner is clamp(i, 0, n-1)
+// and the high corner clamp(i+1, 0, n-1), so the high corner is one stride
+// further exactly when 0 <= i < n-1, else it coincides with the low corner.
+// One multiply per axis instead of two (v_mul_lo_u32 is quarter rate).
+struct CellOffsets {
+    unsigned base, dx, dy, dz;
+};
+
+__device__ __forceinline__ CellOffsets cell_offsets(int ix, int iy, int iz, int X, int Y, int Z, int sx, int sy, int sz)
+{
+    CellOffsets o;
+    o.base = unsigned(iclamp(ix, 0, X - 1)) * unsigned(sx) + unsigned(iclamp(iy, 0, Y - 1)) * unsigned(sy) +
+             unsigned(iclamp(iz, 0, Z - 1)) * unsigned(sz);
+    o.dx = unsigned(ix) < unsigned(X - 1) ? unsigned(sx) : 0u;
+    o.dy = unsigned(iy) < unsigned(Y - 1) ? unsigned(sy) : 0u;
+    o.dz = unsigned(iz) < unsigned(Z - 1) ? unsigned(sz) : 0u;
+    return o;
+}
+
 __device__ __forceinline__ void cell_load(const Vol& V, int ix, int iy, int iz, float d[8])
 {
-    const int lx = iclamp(ix, 0, V.X - 1), hx = iclamp(ix + 1, 0, V.X - 1);
-    const int ly = iclamp(iy, 0, V.Y - 1), hy = iclamp(iy + 1, 0, V.Y - 1);
-    const int lz = iclamp(iz, 0, V.Z - 1), hz = iclamp(iz + 1, 0, V.Z - 1);
-    const int ox0 = lx * V.sx, ox1 = hx * V.sx, oy0 = ly * V.sy, oy1 = hy * V.sy, oz0 = lz * V.sz, oz1 = hz * V.sz;
-    d[0] = V.p[ox0 + oy0 + oz0];
-    d[1] = V.p[ox0 + oy0 + oz1];
-    d[2] = V.p[ox0 + oy1 + oz0];
-    d[3] = V.p[ox0 + oy1 + oz1];
-    d[4] = V.p[ox1 + oy0 + oz0];
-    d[5] = V.p[ox1 + oy0 + oz1];
-    d[6] = V.p[ox1 + oy1 + oz0];
-    d[7] = V.p[ox1 + oy1 + oz1];
+    const CellOffsets o = cell_offsets(ix, iy, iz, V.X, V.Y, V.Z, V.sx, V.sy, V.sz);
+    const unsigned b = o.base, xy = o.dx + o.dy;
+    d[0] = gld(V.p, b);
+    d[1] = gld(V.p, b + o.dz);
+    d[2] = gld(V.p, b + o.dy);
+    d[3] = gld(V.p, b + o.dy + o.dz);
+    d[4] = gld(V.p, b + o.dx);
+    d[5] = gld(V.p, b + o.dx + o.dz);
+    d[6] = gld(V.p, b + xy);
+    d[7] = gld(V.p, b + xy + o.dz);
 }
 
 __device__ __forceinline__ float tri_lerp(const float d[8], float fx, float fy, float fz)
@@ -300,7 +342,7 @@ template <int FILTER>
 __device__ __forceinline__ float fetch(const Vol& V, f3 p, Cell& c)
 {
     if constexpr (FILTER == 0) {
-        return V.p[nearest_offset(V, p)];
+        return gld(V.p, unsigned(nearest_offset(V, p)));
     } else {
         const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
         if (ix != c.ix || iy != c.iy || iz != c.iz) {
@@ -361,9 +403,9 @@ struct FetchPipe<1> {
 template <>
 struct FetchPipe<0> {
     float v, nv;
-    __device__ __forceinline__ void start(const Vol& V, f3 p) { v = V.p[nearest_offset(V, p)]; }
+    __device__ __forceinline__ void start(const Vol& V, f3 p) { v = gld(V.p, unsigned(nearest_offset(V, p))); }
     __device__ __forceinline__ float value(f3) const { return v; }
-    __device__ __forceinline__ void request(const Vol& V, f3 pn) { nv = V.p[nearest_offset(V, pn)]; }
+    __device__ __forceinline__ void request(const Vol& V, f3 pn) { nv = gld(V.p, unsigned(nearest_offset(V, pn))); }
     __device__ __forceinline__ void advance() { v = nv; }
 };
 
@@ -375,6 +417,15 @@ struct Tf {
     const float* p;
     int R, C, sr, sc;
 };
+
+// Columns of a TF table per mode (identity, texture, linear, Gaussian); LDS
+// copies are staged densely, so kernels that read them pass this constant as
+// row stride and the row addressing needs no multiply.
+template <int TFM>
+constexpr int tf_cols()
+{
+    return TFM == 0 ? 2 : (TFM == 1 ? 4 : (TFM == 2 ? 5 : 6));
+}
 
 template <int TFM>
 __device__ __forceinline__ f4 tf_eval(const Tf& T, float density)

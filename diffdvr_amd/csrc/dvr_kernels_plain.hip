@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_adjoint_plain(RenderParams P, AdjointPa
                 int noff = 0;
                 if constexpr (FILTER == 0) {
                     noff = nearest_offset(V, p);
-                    d = V.p[noff];
+                    d = gld(V.p, unsigned(noff));
                 } else {
                     const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
                     if (ix != c.ix || iy != c.iy || iz != c.iz) {

@@ -199,18 +199,16 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
 
     auto flush_cell = [&]() {
         if (rcx != kInvalidCell) {
-            const int lx = iclamp(rcx, 0, V.X - 1), hx = iclamp(rcx + 1, 0, V.X - 1);
-            const int ly = iclamp(rcy, 0, V.Y - 1), hy = iclamp(rcy + 1, 0, V.Y - 1);
-            const int lz = iclamp(rcz, 0, V.Z - 1), hz = iclamp(rcz + 1, 0, V.Z - 1);
-            const int xs0 = lx * avx, xs1 = hx * avx, ys0 = ly * avy, ys1 = hy * avy, zs0 = lz * avz, zs1 = hz * avz;
-            if (rw[0] != 0.f) atomicAdd(avol + xs0 + ys0 + zs0, rw[0]);
-            if (rw[1] != 0.f) atomicAdd(avol + xs0 + ys0 + zs1, rw[1]);
-            if (rw[2] != 0.f) atomicAdd(avol + xs0 + ys1 + zs0, rw[2]);
-            if (rw[3] != 0.f) atomicAdd(avol + xs0 + ys1 + zs1, rw[3]);
-            if (rw[4] != 0.f) atomicAdd(avol + xs1 + ys0 + zs0, rw[4]);
-            if (rw[5] != 0.f) atomicAdd(avol + xs1 + ys0 + zs1, rw[5]);
-            if (rw[6] != 0.f) atomicAdd(avol + xs1 + ys1 + zs0, rw[6]);
-            if (rw[7] != 0.f) atomicAdd(avol + xs1 + ys1 + zs1, rw[7]);
+            const CellOffsets o = cell_offsets(rcx, rcy, rcz, V.X, V.Y, V.Z, avx, avy, avz);
+            const unsigned b = o.base, xy = o.dx + o.dy;
+            if (rw[0] != 0.f) gatomic_add(avol, b, rw[0]);
+            if (rw[1] != 0.f) gatomic_add(avol, b + o.dz, rw[1]);
+            if (rw[2] != 0.f) gatomic_add(avol, b + o.dy, rw[2]);
+            if (rw[3] != 0.f) gatomic_add(avol, b + o.dy + o.dz, rw[3]);
+            if (rw[4] != 0.f) gatomic_add(avol, b + o.dx, rw[4]);
+            if (rw[5] != 0.f) gatomic_add(avol, b + o.dx + o.dz, rw[5]);
+            if (rw[6] != 0.f) gatomic_add(avol, b + xy, rw[6]);
+            if (rw[7] != 0.f) gatomic_add(avol, b + xy + o.dz, rw[7]);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) rw[k] = 0.f;
@@ -285,7 +283,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                                 nearest_index(p, V.X, V.Y, V.Z, gx, gy, gz);
                                 const int goff = gx * avx + gy * avy + gz * avz;
                                 if (goff != nLast) {
-                                    if (nLast >= 0 && nAcc != 0.f) atomicAdd(avol + nLast, nAcc);
+                                    if (nLast >= 0 && nAcc != 0.f) gatomic_add(avol, unsigned(nLast), nAcc);
                                     nLast = goff;
                                     nAcc = 0.f;
                                 }
@@ -322,7 +320,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
     if constexpr (CT_VOL) {
         if (wantVol) {
             if constexpr (FILTER == 0) {
-                if (nLast >= 0 && nAcc != 0.f) atomicAdd(avol + nLast, nAcc);
+                if (nLast >= 0 && nAcc != 0.f) gatomic_add(avol, unsigned(nLast), nAcc);
             } else {
                 flush_cell();
             }
@@ -344,7 +342,8 @@ __global__ __launch_bounds__(kBlock) void k_forward_tiled(RenderParams P, CkptPa
     if (tc.x >= P.W || tc.y >= P.H) return;
     const Ray r = setup_ray(P, tc.x, tc.y, tc.b);
     const Vol V = make_vol(P, tc.b);
-    const Tf T{lds, P.R, P.C, P.C, 1};
+    constexpr int kC = tf_cols<TFM>();
+    const Tf T{lds, P.R, kC, kC, 1};
     float* ck = CK.buf ? CK.buf + (size_t(tc.tile) * CK.cap) * kBlock + threadIdx.x : nullptr;
     Acc acc;
     int n;
@@ -373,14 +372,15 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
     __syncthreads();
 
     SinkTiled<TFM> sink;
-    sink.init(s_g, P.C, usePriv != 0);
+    sink.init(s_g, tf_cols<TFM>(), usePriv != 0);
     float lossLocal = 0.0f;
     const bool valid = tc.x < P.W && tc.y < P.H;
     if (valid) {
         const int b = tc.b, x = tc.x, y = tc.y;
         const Ray r = setup_ray(P, x, y, b);
         const Vol V = make_vol(P, b);
-        const Tf T{s_tf, P.R, P.C, P.C, 1};
+        constexpr int kC = tf_cols<TFM>();
+        const Tf T{s_tf, P.R, kC, kC, 1};
         float* ck = CK.buf + (size_t(tc.tile) * CK.cap) * kBlock + threadIdx.x;
         int n;
         f4 adj_acc;
