@@ -22,7 +22,7 @@ Prints ONE JSON line.  Options:
   --workload c2|c3|c4|c5  --steps K --warmup W  --gpus N (only 1)
   --no-cpu-baseline  --profile (short run for rocprofv3: no plain leg, no baseline)
   --path persistent|fused (c2: persistent plan, or the single-batch fused kernel)
-  --checkpoint-interval K  --wave-width W  (0 = autotuned for c3/c5, defaults for c2/c4)
+  --checkpoint-interval K  --wave-width W  --grid G  (all 0 = autotuned; c4 uses defaults)
   --pmc-probe (3 batched steps and one 256 MiB device copy, for counter calibration)
 """
 from __future__ import annotations
@@ -57,6 +57,7 @@ def parse():
     p.add_argument("--path", default="persistent", choices=["persistent", "fused"])
     p.add_argument("--checkpoint-interval", type=int, default=0)
     p.add_argument("--wave-width", type=int, default=0)
+    p.add_argument("--grid", type=int, default=0, help="persistent workgroups (0 = resident capacity)")
     p.add_argument("--graph-steps", type=int, default=100)
     p.add_argument("--pmc-probe", action="store_true")
     return p.parse_args()
@@ -211,7 +212,11 @@ def bench_c2(args):
                  for b in range(B)]
         plan = bt.BatchPlan([bt.BatchVolume(vol)], [bt.BatchTf(tf, adj_tf[0])], views, volume_filter_mode=1,
                             tf_mode=1, blend_mode=0, loss="l1",
-                            checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8)
+                            checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8,
+                            grid=args.grid)
+
+        if not (args.checkpoint_interval or args.wave_width or args.grid):
+            plan.autotune(iterations=3)
 
         def batched_step():
             plan.run(zero_gradients=True)

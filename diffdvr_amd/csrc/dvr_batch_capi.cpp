@@ -77,6 +77,7 @@ struct dvr_batch_plan {
     double maxSamples = 0;  // upper bound of samples per ray over the views
     // layout
     int K = 16, waveWidth = 8, gridReq = 0, grid = 0, nTiles = 0, cap = 1;
+    int cus = 0, resident = 0;  // compute units, resident workgroups of the current instance
     int qBegin[dvr::kNumXcd + 1] = {};
     size_t lds = 0;
     // device
@@ -135,7 +136,11 @@ struct dvr_batch_plan {
                   "occupancy query");
         hipDeviceProp_t prop;
         check_hip(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-        int g = gridReq > 0 ? gridReq : std::max(1, occ) * prop.multiProcessorCount;
+        cus = prop.multiProcessorCount;
+        resident = std::max(1, occ) * cus;
+        // default: two workgroups per CU (measured best for gather-bound tiles on
+        // MI355X: fewer concurrent rays per CU keep their cells in L1), within residency
+        int g = gridReq > 0 ? gridReq : std::min(resident, 2 * cus);
         g = std::max(1, std::min(g, std::max(nTiles, 1)));
         grid = g;
         cap = std::max(1, std::min((int)std::ceil(maxSamples / K), 1 << 14));
@@ -170,7 +175,10 @@ struct dvr_batch_plan {
 namespace {
 
 std::mutex g_tuneMu;
-std::map<std::string, std::pair<int, int>> g_tuneCache;  // signature -> (K, wave width)
+struct Tuned {
+    int K, ww, grid;
+};
+std::map<std::string, Tuned> g_tuneCache;  // signature -> tuned options
 
 void check_options(const dvr_batch_options* o, int& K, int& ww)
 {
@@ -436,38 +444,59 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
         DeviceGuard g(plan->device);
         hipStream_t s = (hipStream_t)stream;
         const int gridReq = plan->gridReq;
-        std::pair<int, int> best{0, 0};
+        Tuned best{0, 0, 0};
         {
             std::lock_guard<std::mutex> lk(g_tuneMu);
             auto it = g_tuneCache.find(plan->signature);
             if (it != g_tuneCache.end()) best = it->second;
         }
-        if (best.first == 0) {
+        if (best.K == 0) {
             const int its = std::max(1, (int)iterations);
-            const std::vector<int> Ks = plan->gm ? std::vector<int>{8, 16, 32, 64} : std::vector<int>{16};
-            const int wws[] = {8, 16, 4, 32};
             hipEvent_t e0, e1;
             check_hip(hipEventCreate(&e0), "hipEventCreate");
             check_hip(hipEventCreate(&e1), "hipEventCreate");
-            float bestMs = 1e30f;
+            auto time_it = [&](int K, int ww, int g) {
+                check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+                dvr_batch_options o{K, ww, g};
+                set_options(plan, &o);
+                plan->run(0, s);  // warm-up
+                check_hip(hipEventRecord(e0, s), "hipEventRecord");
+                for (int i = 0; i < its; ++i) plan->run(0, s);
+                check_hip(hipEventRecord(e1, s), "hipEventRecord");
+                check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.0f;
+                check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                return ms;
+            };
             try {
-                for (int K : Ks)
-                    for (int ww : wws) {
-                        check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-                        dvr_batch_options o{K, ww, gridReq};
-                        set_options(plan, &o);
-                        plan->run(0, s);  // warm-up
-                        check_hip(hipEventRecord(e0, s), "hipEventRecord");
-                        for (int i = 0; i < its; ++i) plan->run(0, s);
-                        check_hip(hipEventRecord(e1, s), "hipEventRecord");
-                        check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
-                        float ms = 0.0f;
-                        check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-                        if (ms < bestMs) {
-                            bestMs = ms;
-                            best = {K, ww};
-                        }
+                // staged search: workgroups per CU, then wave tile shape, then checkpoint interval
+                Tuned cur{16, 8, 0};
+                float bestMs = 1e30f;
+                {
+                    dvr_batch_options o{cur.K, cur.ww, 0};
+                    set_options(plan, &o);
+                }
+                const int cus = plan->cus, resident = plan->resident;
+                std::vector<int> grids;
+                if (gridReq > 0) grids.push_back(gridReq);
+                else
+                    for (int k = 1; k <= 4; ++k)
+                        if (k * cus <= resident) grids.push_back(k * cus);
+                if (grids.empty()) grids.push_back(resident);
+                for (int g : grids) {
+                    const float ms = time_it(cur.K, cur.ww, g);
+                    if (ms < bestMs) { bestMs = ms; cur.grid = g; }
+                }
+                for (int ww : {8, 16, 4, 32}) {
+                    const float ms = time_it(cur.K, ww, cur.grid);
+                    if (ms < bestMs) { bestMs = ms; cur.ww = ww; }
+                }
+                if (plan->gm)
+                    for (int K : {8, 16, 32, 64}) {
+                        const float ms = time_it(K, cur.ww, cur.grid);
+                        if (ms < bestMs) { bestMs = ms; cur.K = K; }
                     }
+                best = cur;
             } catch (...) {
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);
@@ -479,7 +508,7 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
             g_tuneCache[plan->signature] = best;
         }
         check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-        dvr_batch_options o{best.first, best.second, gridReq};
+        dvr_batch_options o{best.K, best.ww, best.grid};
         set_options(plan, &o);
         plan->zero(true, s);
         if (chosen) {

@@ -172,6 +172,86 @@ struct SinkTiled {
 };
 
 // ---------------------------------------------------------------------------
+// Reverse-walk cell state of the volume and position adjoints.  Consecutive
+// cells of a ray that differ by one step along one axis share the 4 corners
+// of the face between them -- also where the reference clamps corners at the
+// volume border, since the old cell's high corner and the new cell's low
+// corner clamp to the same voxel.  Their gradient accumulators (and corner
+// densities) are kept; only the 4 corners that leave are flushed with global
+// atomics (and only the 4 new ones loaded), which halves the volume-gradient
+// atomics of a ray crossing one face per step.  Corner bit order (x<<2)|(y<<1)|z.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned corner_off(const CellOffsets& o, int k)
+{
+    return o.base + ((k & 4) ? o.dx : 0u) + ((k & 2) ? o.dy : 0u) + ((k & 1) ? o.dz : 0u);
+}
+
+struct CornerCache {
+    int cx, cy, cz;   // current cell, kInvalidCell = none
+    float w[8];       // accumulated corner gradients (volume adjoint)
+    float d[8];       // corner densities (position adjoint)
+};
+
+// Move across the face normal to axis bit B, towards +1 (UP) or -1.
+template <int B, bool UP, bool VOL, bool POS>
+__device__ __forceinline__ void corner_shift(CornerCache& c, const Vol& V, float* avol, const CellOffsets& oldG,
+                                             const CellOffsets& newV)
+{
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k & B) continue;
+        const int f = UP ? k : (k | B);   // leaves; takes the shared corner's state
+        const int s = UP ? (k | B) : k;   // the new cell's fresh corner
+        if constexpr (VOL) {
+            if (c.w[f] != 0.f) gatomic_add(avol, corner_off(oldG, f), c.w[f]);
+            c.w[f] = c.w[s];
+            c.w[s] = 0.f;
+        }
+        if constexpr (POS) {
+            c.d[f] = c.d[s];
+            c.d[s] = gld(V.p, corner_off(newV, s));
+        }
+    }
+}
+
+template <bool VOL, bool POS>
+__device__ __forceinline__ void corner_flush_all(CornerCache& c, const Vol& V, float* avol, int avx, int avy, int avz)
+{
+    if constexpr (VOL) {
+        if (c.cx != kInvalidCell) {
+            const CellOffsets o = cell_offsets(c.cx, c.cy, c.cz, V.X, V.Y, V.Z, avx, avy, avz);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (c.w[k] != 0.f) gatomic_add(avol, corner_off(o, k), c.w[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c.w[k] = 0.f;
+    }
+}
+
+template <bool VOL, bool POS>
+__device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, float* avol, int avx, int avy, int avz, int ix,
+                                            int iy, int iz)
+{
+    const int ddx = ix - c.cx, ddy = iy - c.cy, ddz = iz - c.cz;
+    const bool unit = c.cx != kInvalidCell && (abs(ddx) + abs(ddy) + abs(ddz)) == 1;
+    if (unit) {
+        const CellOffsets oldG = cell_offsets(c.cx, c.cy, c.cz, V.X, V.Y, V.Z, avx, avy, avz);
+        const CellOffsets newV = cell_offsets(ix, iy, iz, V.X, V.Y, V.Z, V.sx, V.sy, V.sz);
+        if (ddx > 0) corner_shift<4, true, VOL, POS>(c, V, avol, oldG, newV);
+        else if (ddx < 0) corner_shift<4, false, VOL, POS>(c, V, avol, oldG, newV);
+        else if (ddy > 0) corner_shift<2, true, VOL, POS>(c, V, avol, oldG, newV);
+        else if (ddy < 0) corner_shift<2, false, VOL, POS>(c, V, avol, oldG, newV);
+        else if (ddz > 0) corner_shift<1, true, VOL, POS>(c, V, avol, oldG, newV);
+        else corner_shift<1, false, VOL, POS>(c, V, avol, oldG, newV);
+    } else {
+        corner_flush_all<VOL, POS>(c, V, avol, avx, avy, avz);
+        if constexpr (POS) cell_load(V, ix, iy, iz, c.d);
+    }
+    c.cx = ix; c.cy = iy; c.cz = iz;
+}
+
+// ---------------------------------------------------------------------------
 // Reverse march of one ray with transmittance checkpoints every K*ckEvery
 // samples, walked back in register segments of K samples.  GM is the set of gradient
 // kinds compiled in: bit0 TF, bit1 volume, bit2 position (camera / step);
@@ -190,29 +270,12 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
     wantPos = CT_POS && wantPos;
     float dv[K], tin[K];
     // reverse-walk cell state for the volume / position adjoint
-    int rcx = kInvalidCell, rcy = 0, rcz = 0;
-    float rw[8], rd[8];
+    CornerCache cc;
+    cc.cx = kInvalidCell; cc.cy = 0; cc.cz = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) { rw[k] = 0.f; rd[k] = 0.f; }
+    for (int k = 0; k < 8; ++k) { cc.w[k] = 0.f; cc.d[k] = 0.f; }
     int nLast = -1;
     float nAcc = 0.0f;
-
-    auto flush_cell = [&]() {
-        if (rcx != kInvalidCell) {
-            const CellOffsets o = cell_offsets(rcx, rcy, rcz, V.X, V.Y, V.Z, avx, avy, avz);
-            const unsigned b = o.base, xy = o.dx + o.dy;
-            if (rw[0] != 0.f) gatomic_add(avol, b, rw[0]);
-            if (rw[1] != 0.f) gatomic_add(avol, b + o.dz, rw[1]);
-            if (rw[2] != 0.f) gatomic_add(avol, b + o.dy, rw[2]);
-            if (rw[3] != 0.f) gatomic_add(avol, b + o.dy + o.dz, rw[3]);
-            if (rw[4] != 0.f) gatomic_add(avol, b + o.dx, rw[4]);
-            if (rw[5] != 0.f) gatomic_add(avol, b + o.dx + o.dz, rw[5]);
-            if (rw[6] != 0.f) gatomic_add(avol, b + xy, rw[6]);
-            if (rw[7] != 0.f) gatomic_add(avol, b + xy + o.dz, rw[7]);
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) rw[k] = 0.f;
-    };
 
     const int nseg = (n + K - 1) / K;
     for (int seg = nseg - 1; seg >= 0; --seg) {
@@ -291,18 +354,18 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                             }
                         } else {
                             const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
-                            if (ix != rcx || iy != rcy || iz != rcz) {
-                                if (wantVol) flush_cell();
-                                rcx = ix; rcy = iy; rcz = iz;
-                                if (wantPos) cell_load(V, ix, iy, iz, rd);
+                            if (ix != cc.cx || iy != cc.cy || iz != cc.cz) {
+                                if (wantVol && wantPos) corner_move<true, true>(cc, V, avol, avx, avy, avz, ix, iy, iz);
+                                else if (wantVol) corner_move<true, false>(cc, V, avol, avx, avy, avz, ix, iy, iz);
+                                else corner_move<false, true>(cc, V, avol, avx, avy, avz, ix, iy, iz);
                             }
                             const float fx = p.x - float(ix), fy = p.y - float(iy), fz = p.z - float(iz);
                             float w[8];
-                            if (wantPos) tri_adjoint<true>(rd, fx, fy, fz, adj_d, w, adj_p);
-                            else tri_adjoint<false>(rd, fx, fy, fz, adj_d, w, adj_p);
+                            if (wantPos) tri_adjoint<true>(cc.d, fx, fy, fz, adj_d, w, adj_p);
+                            else tri_adjoint<false>(cc.d, fx, fy, fz, adj_d, w, adj_p);
                             if (wantVol) {
 #pragma unroll
-                                for (int k = 0; k < 8; ++k) rw[k] += w[k];
+                                for (int k = 0; k < 8; ++k) cc.w[k] += w[k];
                             }
                         }
                         if (wantPos) {
@@ -322,7 +385,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
             if constexpr (FILTER == 0) {
                 if (nLast >= 0 && nAcc != 0.f) gatomic_add(avol, unsigned(nLast), nAcc);
             } else {
-                flush_cell();
+                corner_flush_all<true, false>(cc, V, avol, avx, avy, avz);
             }
         }
     }

@@ -255,7 +255,7 @@ typedef struct dvr_batch {
 typedef struct dvr_batch_options {
     int32_t checkpoint_interval; /* K in {8,16,32,64}; 0 = 16 */
     int32_t wave_width;          /* rays along x of a wave's tile: 4, 8, 16 or 32 (tile 4x16 .. 32x2); 0 = 8 */
-    int32_t grid;                /* persistent workgroups; 0 = resident capacity of the device */
+    int32_t grid;                /* persistent workgroups; 0 = 2 per compute unit (within residency) */
 } dvr_batch_options;
 
 typedef struct dvr_batch_plan dvr_batch_plan;
@@ -271,10 +271,12 @@ dvr_status dvr_batch_plan_zero_gradients(dvr_batch_plan* plan, void* stream);
 dvr_status dvr_batch_plan_get_options(const dvr_batch_plan* plan, dvr_batch_options* options);
 /* Re-layout for new options; may allocate (set-up only, not under capture). */
 dvr_status dvr_batch_plan_set_options(dvr_batch_plan* plan, const dvr_batch_options* options);
-/* Autotuner: times every (K, wave_width) candidate on this plan's own batch
- * (`iterations` timed runs each, HIP events), keeps the fastest and writes it
- * to *chosen.  Results are cached per device and batch signature for the
- * process.  Leaves outputs overwritten and gradients ZEROED. */
+/* Autotuner: times candidates on this plan's own batch (`iterations` timed
+ * runs each, HIP events) in three stages -- workgroups per CU (1..4, unless
+ * options.grid was set), wave tile width {8,16,4,32}, checkpoint interval
+ * {8,16,32,64} -- keeps the fastest and writes it to *chosen.  Results are
+ * cached per device and batch signature for the process.  Leaves outputs
+ * overwritten and gradients ZEROED. */
 dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr_batch_options* chosen, void* stream);
 /* Bytes of device scratch the plan holds (checkpoints, tables). */
 int64_t dvr_batch_plan_workspace_bytes(const dvr_batch_plan* plan);
