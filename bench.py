@@ -39,10 +39,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
 # algorithmic bytes per ray-sample of the batched kernel: 32 B of trilinear corner
-# gathers in the forward march, 32 B again in the segment recompute, plus 32 B of
-# volume-gradient atomics (8 corners) or 32 B of corner reloads for the position
-# (camera) gradient
-BYTES_PER_SAMPLE = {"c2": 64, "c3": 96, "c4": 96, "c5": 96}
+# gathers in the forward march, 32 B again for the adjoint, plus 8 atomic
+# read-modify-writes of 8 B for a volume gradient (TF and camera gradients are
+# privatised in LDS / registers: 0 B per sample)
+BYTES_PER_SAMPLE = {"c2": 64, "c3": 128, "c4": 64, "c5": 128}
 METRIC = "fwd+adjoint ray-samples/s on 1 MI355X, batched vs plain, at 1e-4 rel grad err"
 
 
