@@ -546,6 +546,281 @@ void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* lo
     check_hip(dvr::launch_adjoint_tiled(c.P, A, CK, L, true, false, c.filter, c.tfm, c.blend, s), "fused launch");
 }
 
+// ---------------------------------------------------------------------------
+// forward-mode differentiation (renderer/renderer.cpp:419-658)
+// ---------------------------------------------------------------------------
+
+int64_t numel(const dvr_tensor& t)
+{
+    int64_t n = 1;
+    for (int d = 0; d < t.ndim; ++d) n *= t.sizes[d];
+    return n;
+}
+
+void check_index(int idx, int D, const char* name)
+{
+    if (idx < DVR_NOT_ENABLED || idx >= D)
+        fail(DVR_ERR_INVALID_ARGUMENT, "%s = %d is not a derivative index in [0, D=%d) or -1", name, idx, D);
+}
+
+// Scalar indices of the settings: every enabled one lies in [0, D).
+void check_fwd_indices(const dvr_forward_differences* fd)
+{
+    if (!fd) fail(DVR_ERR_INVALID_ARGUMENT, "differences_settings is NULL");
+    if (fd->D <= 0) fail(DVR_ERR_INVALID_ARGUMENT, "number of derivatives must be positive");
+    check_index(fd->d_stepsize, fd->D, "d_stepsize");
+    static const char* sn[3] = {"d_rayStart.x", "d_rayStart.y", "d_rayStart.z"};
+    static const char* dn[3] = {"d_rayDir.x", "d_rayDir.y", "d_rayDir.z"};
+    for (int c = 0; c < 3; ++c) {
+        check_index(fd->d_ray_start[c], fd->D, sn[c]);
+        check_index(fd->d_ray_dir[c], fd->D, dn[c]);
+    }
+}
+
+// d_tf as a (batch, R, C) int32 device tensor shaped like the TF
+void check_dtf(const dvr_tensor& t, int device, int64_t Bt, int R, int C)
+{
+    check_dim(t, "d_tf", 3); check_device(t, "d_tf", device);
+    check_size(t, "d_tf", 0, Bt); check_size(t, "d_tf", 1, R); check_size(t, "d_tf", 2, C);
+    check_dtype(t, "d_tf", DVR_INT32); check_offsets(t, "d_tf");
+}
+
+// Copies d_tf to the host (stream-ordered) and builds the inverse table
+// entry[b][d] = r*C + c; validates the indices and that no two entries of one
+// batch share an index (a single entry per traced index is what the kernel
+// seeds).
+std::vector<int> tf_entry_table(const dvr_tensor& t, int D, hipStream_t s)
+{
+    const int Bt = (int)t.sizes[0], R = (int)t.sizes[1], C = (int)t.sizes[2];
+    int64_t span = 1;
+    for (int d = 0; d < 3; ++d) span += (t.sizes[d] - 1) * t.strides[d];
+    std::vector<int> host(size_t(std::max<int64_t>(span, 1)));
+    if (Bt * R * C > 0) {
+        check_hip(hipMemcpyAsync(host.data(), t.data, sizeof(int) * size_t(span), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync(d_tf)");
+        check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+    std::vector<int> entry(size_t(Bt) * D, -1);
+    for (int b = 0; b < Bt; ++b)
+        for (int r = 0; r < R; ++r)
+            for (int c = 0; c < C; ++c) {
+                const int d = host[size_t(b * t.strides[0] + r * t.strides[1] + c * t.strides[2])];
+                if (d == DVR_NOT_ENABLED) continue;
+                if (d < DVR_NOT_ENABLED || d >= D)
+                    fail(DVR_ERR_INVALID_ARGUMENT, "d_tf[%d][%d][%d] = %d is not a derivative index in [0, D=%d) or -1",
+                         b, r, c, d, D);
+                int& e = entry[size_t(b) * D + d];
+                if (e >= 0)
+                    fail(DVR_ERR_UNSUPPORTED, "d_tf maps two TF entries of batch %d to derivative index %d", b, d);
+                e = r * C + c;
+            }
+    return entry;
+}
+
+void do_forward_gradients(const dvr_inputs* inputs, const dvr_forward_differences* fd, dvr_outputs* outputs,
+                          dvr_tensor* gout, void* stream)
+{
+    Checked c = check_inputs(inputs);
+    check_fwd_indices(fd);
+    const int D = fd->D;
+    // d_tf is read only when TF derivatives are enabled (renderer.cpp:430-447)
+    if (fd->has_tf_derivatives) check_dtf(fd->d_tf, c.device, inputs->tf.sizes[0], c.P.R, c.P.C);
+    const bool hasStep = fd->d_stepsize >= 0;
+    bool hasCam = false;
+    for (int k = 0; k < 3; ++k) hasCam = hasCam || fd->d_ray_start[k] >= 0 || fd->d_ray_dir[k] >= 0;
+    if (!hasStep && !hasCam && !fd->has_tf_derivatives)
+        fail(DVR_ERR_INVALID_ARGUMENT, "at least one derivative must be activated");
+    check_outputs(outputs, c);
+    if (!gout) fail(DVR_ERR_INVALID_ARGUMENT, "gradients_out is NULL");
+    const dvr_tensor& g = *gout;
+    check_dim(g, "gradients_out", 5); check_device(g, "gradients_out", c.device);
+    check_size(g, "gradients_out", 0, c.B); check_size(g, "gradients_out", 1, c.H);
+    check_size(g, "gradients_out", 2, c.W); check_size(g, "gradients_out", 3, D);
+    check_size(g, "gradients_out", 4, 4); check_dtype(g, "gradients_out", DVR_FLOAT32);
+    check_offsets(g, "gradients_out");
+    if (!tiled_ok(c)) fail(DVR_ERR_UNSUPPORTED, "forward derivatives need R*C <= %d", dvr::kMaxTfLds);
+
+    DeviceGuard guard(c.device);
+    hipStream_t s = (hipStream_t)stream;
+    dvr::FwdGradParams G;
+    std::memset(&G, 0, sizeof(G));
+    G.D = D;
+    G.dStep = fd->d_stepsize;
+    for (int k = 0; k < 3; ++k) { G.dStart[k] = fd->d_ray_start[k]; G.dDir[k] = fd->d_ray_dir[k]; }
+    G.grad = (float*)g.data;
+    G.gSb = (int)g.strides[0]; G.gSy = (int)g.strides[1]; G.gSx = (int)g.strides[2];
+    G.gSd = (int)g.strides[3]; G.gSc = (int)g.strides[4];
+    if (fd->has_tf_derivatives) {
+        const std::vector<int> entry = tf_entry_table(fd->d_tf, D, s);
+        int* dEntry = (int*)workspace(c.device, entry.size() * sizeof(int), s);
+        check_hip(hipMemcpyAsync(dEntry, entry.data(), entry.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(tf entry table)");
+        // the host table must outlive the copy
+        check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+        G.tfEntry = dEntry;
+        G.tfEntrySb = fd->d_tf.sizes[0] == 1 ? 0 : D;
+    }
+    check_hip(dvr::launch_forward_gradients(c.P, G, c.filter, c.tfm, c.blend, s), "forward gradients launch");
+}
+
+void do_fwd_to_adjoint(const dvr_tensor* fvp, const dvr_tensor* acp, const dvr_forward_differences* fd,
+                       dvr_adjoint_outputs* ao, void* stream)
+{
+    if (!fvp) fail(DVR_ERR_INVALID_ARGUMENT, "forward_variables is NULL");
+    const dvr_tensor& fv = *fvp;
+    check_dim(fv, "forwardVariables", 5);
+    const int device = fv.device;
+    if (device < 0) fail(DVR_ERR_UNSUPPORTED, "forwardVariables is a host (CPU) tensor: this library runs on the GPU only");
+    const int B = (int)fv.sizes[0], H = (int)fv.sizes[1], W = (int)fv.sizes[2], D = (int)fv.sizes[3];
+    check_size(fv, "forwardVariables", 4, 4); check_dtype(fv, "forwardVariables", DVR_FLOAT32);
+    check_offsets(fv, "forwardVariables");
+    if (B > 65535) fail(DVR_ERR_UNSUPPORTED, "at most 65535 images per call (batch is %d)", B);
+    if (!acp) fail(DVR_ERR_INVALID_ARGUMENT, "adj_color is NULL");
+    const dvr_tensor& ac = *acp;
+    check_dim(ac, "gradientOutputColor", 4); check_device(ac, "gradientOutputColor", device);
+    check_size(ac, "gradientOutputColor", 0, B); check_size(ac, "gradientOutputColor", 1, H);
+    check_size(ac, "gradientOutputColor", 2, W); check_size(ac, "gradientOutputColor", 3, 4);
+    check_dtype(ac, "gradientOutputColor", DVR_FLOAT32); check_offsets(ac, "gradientOutputColor");
+    check_fwd_indices(fd);
+    if (fd->D != D)
+        fail(DVR_ERR_INVALID_ARGUMENT, "differences_settings.D = %d, but forwardVariables has %d derivatives", fd->D, D);
+    if (!ao) fail(DVR_ERR_INVALID_ARGUMENT, "adj_outputs is NULL");
+    if (ao->has_volume_derivatives) fail(DVR_ERR_INVALID_ARGUMENT, "volume derivatives are not supported with forward variables");
+
+    dvr::FwdToAdjParams F;
+    std::memset(&F, 0, sizeof(F));
+    F.B = B; F.H = H; F.W = W; F.D = D;
+    F.fv = (const float*)fv.data;
+    F.fSb = (int)fv.strides[0]; F.fSy = (int)fv.strides[1]; F.fSx = (int)fv.strides[2];
+    F.fSd = (int)fv.strides[3]; F.fSc = (int)fv.strides[4];
+    F.ac = (const float*)ac.data;
+    F.aSb = (int)ac.strides[0]; F.aSy = (int)ac.strides[1]; F.aSx = (int)ac.strides[2]; F.aSc = (int)ac.strides[3];
+    F.dStep = fd->d_stepsize;
+    for (int k = 0; k < 3; ++k) { F.dStart[k] = fd->d_ray_start[k]; F.dDir[k] = fd->d_ray_dir[k]; }
+    if (ao->has_stepsize_derivatives && F.dStep >= 0) {
+        const dvr_tensor& t = ao->adj_stepsize;
+        check_dim(t, "adj_stepSize", 3); check_device(t, "adj_stepSize", device);
+        check_batch(t, "adj_stepSize", B); check_batch(t, "adj_stepSize", H, 1); check_batch(t, "adj_stepSize", W, 2);
+        if (t.sizes[0] > B || t.sizes[1] > H || t.sizes[2] > W)
+            fail(DVR_ERR_INVALID_ARGUMENT, "adj_stepSize is larger than the forward variables");
+        check_dtype(t, "adj_stepSize", DVR_FLOAT32); check_offsets(t, "adj_stepSize");
+        F.ast = (float*)t.data;
+        F.astSb = bstride(t, 0); F.astSy = bstride(t, 1); F.astSx = bstride(t, 2);
+        F.astAtomic = (t.sizes[0] != B || t.sizes[1] != H || t.sizes[2] != W) ? 1 : 0;
+    }
+    bool cam = false;
+    for (int k = 0; k < 3; ++k) cam = cam || F.dStart[k] >= 0 || F.dDir[k] >= 0;
+    if (ao->has_camera_derivatives && cam) {
+        const dvr_tensor& s = ao->adj_camera_ray_start;
+        const dvr_tensor& d = ao->adj_camera_ray_dir;
+        const char* nm[2] = {"adj_cameraRayStart", "adj_cameraRayDir"};
+        const dvr_tensor* ts[2] = {&s, &d};
+        for (int i = 0; i < 2; ++i) {
+            const dvr_tensor& t = *ts[i];
+            check_dim(t, nm[i], 4); check_device(t, nm[i], device);
+            check_batch(t, nm[i], B);
+            if (t.sizes[0] > B) fail(DVR_ERR_INVALID_ARGUMENT, "%s has a larger batch than the forward variables", nm[i]);
+            check_size(t, nm[i], 1, H); check_size(t, nm[i], 2, W); check_size(t, nm[i], 3, 3);
+            check_dtype(t, nm[i], DVR_FLOAT32); check_offsets(t, nm[i]);
+        }
+        F.ars = (float*)s.data;
+        F.arsSb = bstride(s, 0); F.arsSy = (int)s.strides[1]; F.arsSx = (int)s.strides[2]; F.arsSc = (int)s.strides[3];
+        F.ard = (float*)d.data;
+        F.ardSb = bstride(d, 0); F.ardSy = (int)d.strides[1]; F.ardSx = (int)d.strides[2]; F.ardSc = (int)d.strides[3];
+        F.camAtomic = (s.sizes[0] != B || d.sizes[0] != B) ? 1 : 0;
+    }
+    const int* dtf = nullptr;
+    int dSb = 0, dSr = 0, dSc = 0, tSb = 0, tSr = 0, tSc = 0, R = 0, C = 0;
+    float* atf = nullptr;
+    if (ao->has_tf_derivatives && fd->has_tf_derivatives) {
+        const dvr_tensor& t = ao->adj_tf;
+        check_dim(t, "adj_tf", 3); check_device(t, "adj_tf", device);
+        check_batch(t, "adj_tf", B);
+        if (t.sizes[0] > B) fail(DVR_ERR_INVALID_ARGUMENT, "adj_tf has a larger batch than the forward variables");
+        check_dtype(t, "adj_tf", DVR_FLOAT32); check_offsets(t, "adj_tf");
+        R = (int)t.sizes[1]; C = (int)t.sizes[2];
+        const dvr_tensor& dt = fd->d_tf;
+        check_dim(dt, "d_tf", 3);
+        check_batch(dt, "d_tf", B);
+        if (dt.sizes[0] > B) fail(DVR_ERR_INVALID_ARGUMENT, "d_tf has a larger batch than the forward variables");
+        check_dtf(dt, device, dt.sizes[0], R, C);
+        if (D > dvr::kMaxFwdSums)
+            fail(DVR_ERR_UNSUPPORTED, "TF contraction supports D <= %d derivatives", dvr::kMaxFwdSums);
+        dtf = (const int*)dt.data;
+        dSb = bstride(dt, 0); dSr = (int)dt.strides[1]; dSc = (int)dt.strides[2];
+        atf = (float*)t.data;
+        tSb = bstride(t, 0); tSr = (int)t.strides[1]; tSc = (int)t.strides[2];
+    }
+    if (!F.ast && !F.ars && !atf) fail(DVR_ERR_NO_GRADIENT, "At least one adjoint variable must be activated!");
+    DeviceGuard guard(device);
+    hipStream_t s = (hipStream_t)stream;
+    if (atf) F.sums = (float*)workspace(device, sizeof(float) * size_t(B) * D, s);
+    check_hip(dvr::launch_forward_to_adjoint(F, dtf, dSb, dSr, dSc, atf, tSb, tSr, tSc, R, C, s),
+              "forward variables to gradients launch");
+}
+
+void do_compare(const dvr_tensor* colp, const dvr_tensor* gp, const dvr_tensor* refp, dvr_tensor* dop, dvr_tensor* gop,
+                int reduce, void* stream)
+{
+    if (!colp || !gp || !refp || !dop || !gop) fail(DVR_ERR_INVALID_ARGUMENT, "a tensor argument is NULL");
+    const dvr_tensor& col = *colp;
+    check_dim(col, "colorInput", 4); check_dtype(col, "colorInput", DVR_FLOAT32);
+    const int device = col.device;
+    if (device < 0) fail(DVR_ERR_UNSUPPORTED, "colorInput is a host (CPU) tensor: this library runs on the GPU only");
+    const int B = (int)col.sizes[0], H = (int)col.sizes[1], W = (int)col.sizes[2];
+    check_size(col, "colorInput", 3, 4); check_offsets(col, "colorInput");
+    if (B > 65535) fail(DVR_ERR_UNSUPPORTED, "at most 65535 images per call (batch is %d)", B);
+    const dvr_tensor& g = *gp;
+    check_dim(g, "gradientsInput", 5); check_device(g, "gradientsInput", device);
+    check_size(g, "gradientsInput", 0, B); check_size(g, "gradientsInput", 1, H); check_size(g, "gradientsInput", 2, W);
+    const int D = (int)g.sizes[3];
+    check_size(g, "gradientsInput", 4, 4); check_dtype(g, "gradientsInput", DVR_FLOAT32);
+    check_offsets(g, "gradientsInput");
+    const dvr_tensor& ref = *refp;
+    check_dim(ref, "colorReference", 4); check_device(ref, "colorReference", device);
+    check_size(ref, "colorReference", 0, B); check_size(ref, "colorReference", 1, H);
+    check_size(ref, "colorReference", 2, W); check_size(ref, "colorReference", 3, 4);
+    check_dtype(ref, "colorReference", DVR_FLOAT32); check_offsets(ref, "colorReference");
+    const dvr_tensor& dout = *dop;
+    const dvr_tensor& go = *gop;
+    dvr::CompareParams Q;
+    std::memset(&Q, 0, sizeof(Q));
+    if (reduce) {
+        check_dim(dout, "differenceOut", 1); check_device(dout, "differenceOut", device);
+        check_size(dout, "differenceOut", 0, 1); check_dtype(dout, "differenceOut", DVR_FLOAT32);
+        check_dim(go, "gradientsOut", 1); check_device(go, "gradientsOut", device);
+        check_size(go, "gradientsOut", 0, D); check_dtype(go, "gradientsOut", DVR_FLOAT32);
+        check_offsets(go, "gradientsOut");
+        Q.qSd = (int)go.strides[0];
+    } else {
+        check_dim(dout, "differenceOut", 4); check_device(dout, "differenceOut", device);
+        check_size(dout, "differenceOut", 0, B); check_size(dout, "differenceOut", 1, H);
+        check_size(dout, "differenceOut", 2, W); check_size(dout, "differenceOut", 3, 1);
+        check_dtype(dout, "differenceOut", DVR_FLOAT32); check_offsets(dout, "differenceOut");
+        check_dim(go, "gradientsOut", 4); check_device(go, "gradientsOut", device);
+        check_size(go, "gradientsOut", 0, B); check_size(go, "gradientsOut", 1, H);
+        check_size(go, "gradientsOut", 2, W); check_size(go, "gradientsOut", 3, D);
+        check_dtype(go, "gradientsOut", DVR_FLOAT32); check_offsets(go, "gradientsOut");
+        Q.oSb = (int)dout.strides[0]; Q.oSy = (int)dout.strides[1]; Q.oSx = (int)dout.strides[2];
+        Q.qSb = (int)go.strides[0]; Q.qSy = (int)go.strides[1]; Q.qSx = (int)go.strides[2]; Q.qSd = (int)go.strides[3];
+    }
+    Q.B = B; Q.H = H; Q.W = W; Q.D = D;
+    Q.col = (const float*)col.data;
+    Q.cSb = (int)col.strides[0]; Q.cSy = (int)col.strides[1]; Q.cSx = (int)col.strides[2]; Q.cSc = (int)col.strides[3];
+    Q.gin = (const float*)g.data;
+    Q.gSb = (int)g.strides[0]; Q.gSy = (int)g.strides[1]; Q.gSx = (int)g.strides[2];
+    Q.gSd = (int)g.strides[3]; Q.gSc = (int)g.strides[4];
+    Q.ref = (const float*)ref.data;
+    Q.rSb = (int)ref.strides[0]; Q.rSy = (int)ref.strides[1]; Q.rSx = (int)ref.strides[2]; Q.rSc = (int)ref.strides[3];
+    Q.dout = (float*)dout.data;
+    Q.gout = (float*)go.data;
+    Q.reduce = reduce ? 1 : 0;
+    const double n = double(B) * H * W;
+    Q.norm = n > 0 ? float(1.0 / n) : 0.0f;
+    DeviceGuard guard(device);
+    check_hip(dvr::launch_compare_to_image(Q, (hipStream_t)stream), "compareToImage launch");
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -590,6 +865,25 @@ dvr_status dvr_render_fused(const dvr_inputs* inputs, dvr_outputs* outputs, cons
                             dvr_adjoint_outputs* adj_outputs, int32_t checkpoint_interval, void* stream)
 {
     return guarded([&] { do_fused(inputs, outputs, loss, adj_outputs, checkpoint_interval, stream); });
+}
+
+dvr_status dvr_render_forward_gradients(const dvr_inputs* inputs, const dvr_forward_differences* settings,
+                                        dvr_outputs* outputs, dvr_tensor* gradients_out, void* stream)
+{
+    return guarded([&] { do_forward_gradients(inputs, settings, outputs, gradients_out, stream); });
+}
+
+dvr_status dvr_forward_variables_to_gradients(const dvr_tensor* forward_variables, const dvr_tensor* adj_color,
+                                              const dvr_forward_differences* settings,
+                                              dvr_adjoint_outputs* adj_outputs, void* stream)
+{
+    return guarded([&] { do_fwd_to_adjoint(forward_variables, adj_color, settings, adj_outputs, stream); });
+}
+
+dvr_status dvr_compare_to_image(const dvr_tensor* color, const dvr_tensor* gradients, const dvr_tensor* reference,
+                                dvr_tensor* difference_out, dvr_tensor* gradients_out, int32_t reduce, void* stream)
+{
+    return guarded([&] { do_compare(color, gradients, reference, difference_out, gradients_out, reduce, stream); });
 }
 
 int64_t dvr_checkpoint_buffer_elements(const dvr_inputs* inputs, int32_t checkpoint_interval)

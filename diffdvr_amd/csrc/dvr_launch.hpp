@@ -65,4 +65,11 @@ hipError_t launch_adam(const AdamSeg* dSegs, int nSegs, long long maxN, long lon
 hipError_t launch_batched(const BatchParams& P, int filter, int tfm, int blend, int K, int gm, int grid, size_t lds,
                           hipStream_t s, int* occ);
 
+// forward-mode derivatives (dvr_kernels_fwdgrad.hip)
+hipError_t launch_forward_gradients(const RenderParams& P, const FwdGradParams& G, int filter, int tfm, int blend,
+                                    hipStream_t s);
+hipError_t launch_forward_to_adjoint(const FwdToAdjParams& F, const int* dtf, int dSb, int dSr, int dSc, float* atf,
+                                     int tSb, int tSr, int tSc, int R, int C, hipStream_t s);
+hipError_t launch_compare_to_image(const CompareParams& Q, hipStream_t s);
+
 }  // namespace dvr

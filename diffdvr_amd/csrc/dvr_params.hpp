@@ -148,4 +148,47 @@ struct AdamSeg {
     float lo, hi;               // clamp after the update when lo <= hi
 };
 
+// ---------------------------------------------------------------------------
+// Forward-mode differentiation (renderer/renderer_kernels.cuh:135-308, :504-598)
+// ---------------------------------------------------------------------------
+constexpr int kFwdChunk = 8;      // derivative indices traced per launch slice
+constexpr int kMaxFwdSums = 16384; // D bound of the LDS pixel contraction
+
+struct FwdGradParams {
+    int D;
+    int dStep;                 // derivative index of the step size or -1
+    int dStart[3], dDir[3];    // derivative indices of ray start / direction components or -1
+    const int* tfEntry;        // [batch][D]: r*C + c of the TF entry traced by index d, or -1 (NULL = none)
+    int tfEntrySb;             // D, or 0 when one table serves every batch
+    float* grad;               // B x H x W x D x 4
+    int gSb, gSy, gSx, gSd, gSc;
+};
+
+// forward variables -> adjoint variables.  The per-pixel outputs are
+// accumulated (+=); an output that broadcasts over the batch or image
+// (stride 0) takes atomic adds.
+struct FwdToAdjParams {
+    int B, H, W, D;
+    const float* fv; int fSb, fSy, fSx, fSd, fSc;      // B x H x W x D x 4
+    const float* ac; int aSb, aSy, aSx, aSc;           // B x H x W x 4
+    int dStep, dStart[3], dDir[3];
+    float* ast; int astSb, astSy, astSx, astAtomic;    // step-size adjoint or NULL
+    float* ars; int arsSb, arsSy, arsSx, arsSc;        // ray start adjoint or NULL
+    float* ard; int ardSb, ardSy, ardSx, ardSc;        // ray direction adjoint or NULL
+    int camAtomic;
+    float* sums;                                       // [B][D] contraction over pixels (zeroed here), or NULL
+};
+
+// compareToImage (renderer/renderer.cpp:562-658)
+struct CompareParams {
+    int B, H, W, D;
+    const float* col; int cSb, cSy, cSx, cSc;
+    const float* gin; int gSb, gSy, gSx, gSd, gSc;
+    const float* ref; int rSb, rSy, rSx, rSc;
+    float* dout; int oSb, oSy, oSx;        // B x H x W x 1 (reduce: 1 element)
+    float* gout; int qSb, qSy, qSx, qSd;   // B x H x W x D (reduce: D elements, stride qSd)
+    int reduce;
+    float norm;                            // 1 / (B*H*W)
+};
+
 }  // namespace dvr

@@ -223,6 +223,19 @@ dvr_outputs pack_outputs(const RendererOutputs& o)
     return out;
 }
 
+dvr_forward_differences pack_differences(const ForwardDifferencesSettings& f)
+{
+    dvr_forward_differences d;
+    std::memset(&d, 0, sizeof(d));
+    d.D = f.D;
+    d.d_stepsize = f.d_stepsize;
+    d.d_ray_start[0] = f.d_rayStart.x; d.d_ray_start[1] = f.d_rayStart.y; d.d_ray_start[2] = f.d_rayStart.z;
+    d.d_ray_dir[0] = f.d_rayDir.x; d.d_ray_dir[1] = f.d_rayDir.y; d.d_ray_dir[2] = f.d_rayDir.z;
+    d.has_tf_derivatives = f.has_tf_derivatives ? 1 : 0;
+    if (f.has_tf_derivatives) d.d_tf = to_tensor(f.d_tf);
+    return d;
+}
+
 dvr_adjoint_outputs pack_adjoint(const AdjointOutputs& a)
 {
     dvr_adjoint_outputs o;
@@ -561,6 +574,34 @@ PYBIND11_MODULE(_pyrenderer, m)
           },
           py::arg("inputs"), py::arg("outputs"), py::arg("loss"), py::arg("target"), py::arg("adj_outputs"),
           py::arg("loss_out") = py::none(), py::arg("scale") = 0.0f, py::arg("checkpoint_interval") = 0);
+    m.def("render_forward_gradients",
+          [](const RendererInputs& inputs, const ForwardDifferencesSettings& fds, RendererOutputs& outputs,
+             py::object gradients_out) {
+              PackedInputs p = pack_inputs(inputs);
+              dvr_outputs o = pack_outputs(outputs);
+              dvr_forward_differences f = pack_differences(fds);
+              dvr_tensor g = to_tensor(gradients_out);
+              check(dvr_render_forward_gradients(&p.in, &f, &o, &g, current_stream(p.in.volume.device)));
+          },
+          py::arg("inputs"), py::arg("differences_settings"), py::arg("outputs"), py::arg("gradients_out"));
+    m.def("forward_variables_to_gradients",
+          [](py::object forward_variables, py::object adj_color, const ForwardDifferencesSettings& fds,
+             AdjointOutputs& adj) {
+              dvr_tensor fv = to_tensor(forward_variables), ac = to_tensor(adj_color);
+              dvr_forward_differences f = pack_differences(fds);
+              dvr_adjoint_outputs ao = pack_adjoint(adj);
+              check(dvr_forward_variables_to_gradients(&fv, &ac, &f, &ao, current_stream(fv.device)));
+          },
+          py::arg("forward_variables"), py::arg("adj_color"), py::arg("differences_settings"), py::arg("adj_outputs"));
+    m.def("compare_to_image",
+          [](py::object color, py::object gradients, py::object reference, py::object difference_out,
+             py::object gradients_out, bool reduce) {
+              dvr_tensor c = to_tensor(color), g = to_tensor(gradients), r = to_tensor(reference);
+              dvr_tensor d = to_tensor(difference_out), go = to_tensor(gradients_out);
+              check(dvr_compare_to_image(&c, &g, &r, &d, &go, reduce ? 1 : 0, current_stream(c.device)));
+          },
+          py::arg("color_input"), py::arg("gradients_input"), py::arg("color_reference"), py::arg("difference_out"),
+          py::arg("gradients_out"), py::arg("reduce"));
     m.def("checkpoint_buffer_elements",
           [](const RendererInputs& inputs, int K) {
               PackedInputs p = pack_inputs(inputs);

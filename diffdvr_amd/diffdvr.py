@@ -10,7 +10,9 @@ Differences to the reference:
     every ``checkpoint_interval`` samples into a buffer saved for backward, and
     the backward launch recomputes each segment from it (no blend inversion);
     ``adjoint_method="inversion"`` selects the reference's scheme;
-  * ``gradient_method="forward"`` (forward-mode dual numbers) is not available.
+  * ``gradient_method="forward"`` traces the derivatives in launch slices of
+    eight indices (any D) instead of one dual number of width D; the
+    contraction with the output adjoint accumulates into the gradients.
 """
 from __future__ import annotations
 
@@ -95,16 +97,40 @@ def size_of_tensors(*tensors):
     return sum(t.element_size() * t.numel() for t in tensors if t is not None)
 
 
+def assemble_forward_differences(optimize_camera, optimize_tf, optimize_volume, tf):
+    """Derivative indices of the forward mode (pytests/diffdvr/renderer.py:61-80):
+    camera ray start 0-2, ray direction 3-5, then every TF entry.  d_tf has the
+    TF's batch size, so per-batch TFs are traced per batch."""
+    s = pyrenderer.ForwardDifferencesSettings()
+    D = 0
+    if optimize_camera:
+        s.d_rayStart = pyrenderer.int3(D, D + 1, D + 2)
+        s.d_rayDir = pyrenderer.int3(D + 3, D + 4, D + 5)
+        D += 6
+    if optimize_tf:
+        s.has_tf_derivatives = True
+        B, R, C = tf.shape
+        d_tf = torch.arange(D, D + R * C, dtype=torch.int32, device=tf.device).reshape(1, R, C)
+        s.d_tf = d_tf.expand(B, R, C).contiguous()
+        D += R * C
+    if optimize_volume:
+        raise ValueError("forward mode does not support volume derivatives")
+    s.D = D
+    return s
+
+
 class _RendererFunction(torch.autograd.Function):
-    """Forward render + adjoint backward (pytests/diffdvr/renderer.py:61-261)."""
+    """Forward render + adjoint or forward-mode backward (pytests/diffdvr/renderer.py:61-261)."""
 
     @staticmethod
     def forward(ctx, camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, inputs, fov_y_radians,
                 optimize_camera, optimize_tf, optimize_volume, use_adjoint_method, forward_delayed_gradients,
                 tf_delayed_accumulation, profiling, adjoint_method, checkpoint_interval):
         if not use_adjoint_method:
-            raise NotImplementedError("gradient_method='forward' (forward-mode differentiation) is not available "
-                                      "in this build; use gradient_method='adjoint'")
+            return _RendererFunction._forward_mode(ctx, camera_matrix, camera_ray_start, camera_ray_dir, tf, volume,
+                                                   inputs, fov_y_radians, optimize_camera, optimize_tf,
+                                                   optimize_volume, forward_delayed_gradients, profiling)
+        ctx.use_adjoint_method = True
         ctx.inputs = inputs
         ctx.fov_y_radians = fov_y_radians
         ctx.flags = (optimize_camera, optimize_tf, optimize_volume, tf_delayed_accumulation)
@@ -144,7 +170,92 @@ class _RendererFunction(torch.autograd.Function):
         return color
 
     @staticmethod
+    def _set_inputs(inputs, camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, fov_y_radians):
+        if inputs.camera_mode == pyrenderer.CameraMode.RayStartDir:
+            inputs.camera = pyrenderer.CameraPerPixelRays(camera_ray_start, camera_ray_dir)
+        elif camera_matrix is not None:
+            inputs.camera = pyrenderer.CameraReferenceFrame(camera_matrix, fov_y_radians)
+        inputs.tf = tf
+        inputs.volume = volume
+
+    @staticmethod
+    def _forward_mode(ctx, camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, inputs, fov_y_radians,
+                      optimize_camera, optimize_tf, optimize_volume, forward_delayed_gradients, profiling):
+        """gradient_method="forward" (pytests/diffdvr/renderer.py:139-158)."""
+        ctx.use_adjoint_method = False
+        ctx.inputs = inputs
+        ctx.fov_y_radians = fov_y_radians
+        ctx.flags = (optimize_camera, optimize_tf, optimize_volume, False)
+        ctx.profiling = profiling
+        ctx.forward_delayed_gradients = forward_delayed_gradients
+        assert implies(optimize_camera, inputs.camera_mode == pyrenderer.CameraMode.RayStartDir)
+        _RendererFunction._set_inputs(inputs, camera_matrix, camera_ray_start, camera_ray_dir, tf, volume,
+                                      fov_y_radians)
+        tensors_for_batches = [camera_matrix, camera_ray_start, camera_ray_dir, tf, volume]
+        B = max([(1 if t is None else t.shape[0]) for t in tensors_for_batches])
+        W, H = inputs.screen_size.x, inputs.screen_size.y
+        color = torch.empty(B, H, W, 4, dtype=volume.dtype, device=volume.device)
+        term = torch.empty(B, H, W, dtype=torch.int32, device=volume.device)
+        outputs = pyrenderer.RendererOutputs(color, term)
+        fwd_grads = None
+        with Timer(profiling is not None, volume.is_cuda) as timer:
+            if forward_delayed_gradients:
+                # derivatives are traced in backward
+                pyrenderer.Renderer.render_forward(inputs, outputs)
+            else:
+                ctx.differences_settings = assemble_forward_differences(optimize_camera, optimize_tf,
+                                                                        optimize_volume, tf)
+                fwd_grads = torch.zeros(B, H, W, ctx.differences_settings.D, 4, dtype=volume.dtype,
+                                        device=volume.device)
+                pyrenderer.Renderer.render_forward_gradients(inputs, ctx.differences_settings, outputs, fwd_grads)
+        if profiling is not None:
+            profiling.forward_ms += timer.elapsed_ms()
+            profiling.forward_bytes += size_of_tensors(camera_matrix, camera_ray_start, camera_ray_dir, tf, volume)
+        ctx.save_for_backward(camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, fwd_grads)
+        return color
+
+    @staticmethod
+    def _backward_forward_mode(ctx, grad_output_color):
+        """pytests/diffdvr/renderer.py:221-258."""
+        camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, fwd_grads = ctx.saved_tensors
+        optimize_camera, optimize_tf, optimize_volume, _ = ctx.flags
+        inputs = ctx.inputs
+        profiling = ctx.profiling
+        B, H, W, _ = grad_output_color.shape
+        with Timer(profiling is not None, volume.is_cuda) as timer:
+            if ctx.forward_delayed_gradients:
+                _RendererFunction._set_inputs(inputs, camera_matrix, camera_ray_start, camera_ray_dir, tf, volume,
+                                              ctx.fov_y_radians)
+                settings = assemble_forward_differences(optimize_camera, optimize_tf, optimize_volume, tf)
+                color = torch.empty(B, H, W, 4, dtype=volume.dtype, device=volume.device)
+                term = torch.empty(B, H, W, dtype=torch.int32, device=volume.device)
+                fwd_grads = torch.zeros(B, H, W, settings.D, 4, dtype=volume.dtype, device=volume.device)
+                pyrenderer.Renderer.render_forward_gradients(inputs, settings, pyrenderer.RendererOutputs(color, term),
+                                                             fwd_grads)
+            else:
+                settings = ctx.differences_settings
+            grad_camera_ray_start = grad_camera_ray_dir = grad_tf = None
+            adj = pyrenderer.AdjointOutputs()
+            if optimize_camera:
+                grad_camera_ray_start = torch.zeros_like(camera_ray_start)
+                grad_camera_ray_dir = torch.zeros_like(camera_ray_dir)
+                adj.has_camera_derivatives = True
+                adj.adj_camera_ray_start = grad_camera_ray_start
+                adj.adj_camera_ray_dir = grad_camera_ray_dir
+            if optimize_tf:
+                grad_tf = torch.zeros_like(tf)
+                adj.has_tf_derivatives = True
+                adj.adj_tf = grad_tf
+            pyrenderer.Renderer.forward_variables_to_gradients(fwd_grads, grad_output_color.contiguous(), settings,
+                                                               adj)
+        if profiling is not None:
+            profiling.backward_ms += timer.elapsed_ms()
+        return (None, grad_camera_ray_start, grad_camera_ray_dir, grad_tf, None) + (None,) * 11
+
+    @staticmethod
     def backward(ctx, grad_output_color):
+        if not ctx.use_adjoint_method:
+            return _RendererFunction._backward_forward_mode(ctx, grad_output_color)
         camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, color, term, checkpoints = ctx.saved_tensors
         optimize_camera, optimize_tf, optimize_volume, tf_delayed_accumulation = ctx.flags
         inputs = ctx.inputs

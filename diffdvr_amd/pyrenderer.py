@@ -102,14 +102,28 @@ class Renderer:
         return _n.checkpoint_buffer_elements(inputs, checkpoint_interval)
 
     @staticmethod
-    def render_forward_gradients(inputs, differences_settings, outputs, gradients_out):
-        raise NotImplementedError("forward-mode differentiation (render_forward_gradients) is not available in "
-                                  "this build; use render_adjoint")
+    def render_forward_gradients(inputs: RendererInputs, differences_settings: ForwardDifferencesSettings,
+                                 outputs: RendererOutputs, gradients_out: torch.Tensor) -> None:
+        """Forward rendering with forward-mode derivatives (renderer/renderer.h:166-196).
+
+        outputs receive what render_forward writes; gradients_out (B*H*W*D*4)
+        the derivative of every pixel's RGBA w.r.t. each of the D traced inputs.
+        """
+        _n.render_forward_gradients(inputs, differences_settings, outputs, gradients_out)
 
     @staticmethod
-    def forward_variables_to_gradients(forward_variables, adj_color, differences_settings, adj_outputs):
-        raise NotImplementedError("forward-mode differentiation (forward_variables_to_gradients) is not available "
-                                  "in this build; use render_adjoint")
+    def forward_variables_to_gradients(forward_variables: torch.Tensor, adj_color: torch.Tensor,
+                                       differences_settings: ForwardDifferencesSettings,
+                                       adj_outputs: AdjointOutputs) -> None:
+        """Contracts forward variables with the colour adjoint into adj_outputs
+        (renderer/renderer.h:198-215); gradients are ACCUMULATED."""
+        _n.forward_variables_to_gradients(forward_variables, adj_color, differences_settings, adj_outputs)
+
+    @staticmethod
+    def compare_to_image(color_input: torch.Tensor, gradients_input: torch.Tensor, color_reference: torch.Tensor,
+                         difference_out: torch.Tensor, gradients_out: torch.Tensor, reduce: bool) -> None:
+        """L2 distance to a reference image with forward derivatives (renderer/renderer.h:220-248)."""
+        _n.compare_to_image(color_input, gradients_input, color_reference, difference_out, gradients_out, reduce)
 
 
 # ---------------------------------------------------------------------------

@@ -178,6 +178,50 @@ dvr_status dvr_render_fused(const dvr_inputs* inputs, dvr_outputs* outputs, cons
  * and interval K (0 if K == 0). */
 int64_t dvr_checkpoint_buffer_elements(const dvr_inputs* inputs, int32_t checkpoint_interval);
 
+/* ---- forward-mode differentiation ---------------------------------------- */
+
+#define DVR_NOT_ENABLED (-1)
+
+/* Mirrors renderer::ForwardDifferencesSettingsHost (renderer/renderer.h:74-93):
+ * which derivative index 0 <= d < D traces which scalar input. */
+typedef struct dvr_forward_differences {
+    int32_t D;                  /* number of traced derivatives, > 0 */
+    int32_t d_stepsize;         /* index of the step size or DVR_NOT_ENABLED */
+    int32_t d_ray_start[3];     /* indices of the evaluated ray start (x,y,z) */
+    int32_t d_ray_dir[3];       /* indices of the evaluated, normalised ray direction */
+    int32_t has_tf_derivatives;
+    dvr_tensor d_tf;            /* int32, same shape as the TF: index of each entry or -1 */
+} dvr_forward_differences;
+
+/* Renderer::renderForwardGradients (renderer/renderer.h:189-196;
+ * renderer/renderer.cpp:419-498).  Writes the colour and termination index
+ * (equal to dvr_render_forward) and gradients_out (B x H x W x D x 4), the
+ * derivative of every pixel's RGBA w.r.t. every traced input.  Indices must
+ * lie in [0, D) and no two TF entries may share one index.  Volume
+ * derivatives are not available in forward mode (as in the reference). */
+dvr_status dvr_render_forward_gradients(const dvr_inputs* inputs, const dvr_forward_differences* settings,
+                                        dvr_outputs* outputs, dvr_tensor* gradients_out, void* stream);
+
+/* Renderer::forwardVariablesToGradients (renderer/renderer.h:198-215;
+ * renderer/renderer.cpp:500-560): contracts forward_variables (B x H x W x D x 4)
+ * with adj_color (B x H x W x 4) into the adjoint outputs for step size,
+ * camera rays and TF.  All outputs are accumulated into (the reference
+ * overwrites the step and camera entries; with the zero-initialised outputs
+ * it is called with the results agree); outputs that broadcast over a
+ * dimension receive the sum over it. has_volume_derivatives must be 0. */
+dvr_status dvr_forward_variables_to_gradients(const dvr_tensor* forward_variables, const dvr_tensor* adj_color,
+                                              const dvr_forward_differences* settings,
+                                              dvr_adjoint_outputs* adj_outputs, void* stream);
+
+/* Renderer::compareToImage (renderer/renderer.h:220-248; renderer/renderer.cpp:562-658): per pixel the
+ * Euclidean RGBA distance L = |color - reference| and its forward derivatives
+ * dot(color - reference, gradients[..., d, :]) / L (0 where L == 0).
+ * reduce != 0: difference_out (1) and gradients_out (D) receive the mean over
+ * all B*H*W pixels; else difference_out is B x H x W x 1 and gradients_out
+ * B x H x W x D.  Any D (the reference instantiates D == 1 only). */
+dvr_status dvr_compare_to_image(const dvr_tensor* color, const dvr_tensor* gradients, const dvr_tensor* reference,
+                                dvr_tensor* difference_out, dvr_tensor* gradients_out, int32_t reduce, void* stream);
+
 /* ---- camera (replaces renderer/camera_impl.cu:68-93 and :302-357) ------- */
 
 /* Camera::generateRays forward: viewport B x 3 x 3 (eye, right, up) -> per-pixel

@@ -177,3 +177,99 @@ def test_adam_group_validation():
         n._AdamGroup([(0x1000, 0, 0x2000, 0x3000, 1, 4, 4, 1.0, 0.0)], 0)
     with pytest.raises(RuntimeError, match="invalid shape"):
         n._AdamGroup([(0x1000, 0x1100, 0x2000, 0x3000, 2, 5, 4, 1.0, 0.0)], 0)
+
+
+# ---------------------------------------------------------------------------
+# forward-mode differentiation: validation happens before any HIP call
+# ---------------------------------------------------------------------------
+class FwdDiff(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_int32), ("d_stepsize", ctypes.c_int32), ("d_ray_start", ctypes.c_int32 * 3),
+                ("d_ray_dir", ctypes.c_int32 * 3), ("has_tf_derivatives", ctypes.c_int32), ("d_tf", Tensor)]
+
+
+class AdjOut(ctypes.Structure):
+    _fields_ = [("has_volume_derivatives", ctypes.c_int32), ("adj_volume", Tensor),
+                ("has_stepsize_derivatives", ctypes.c_int32), ("adj_stepsize", Tensor),
+                ("has_camera_derivatives", ctypes.c_int32), ("adj_camera_ray_start", Tensor),
+                ("adj_camera_ray_dir", Tensor), ("has_tf_derivatives", ctypes.c_int32), ("adj_tf", Tensor),
+                ("tf_delayed_accumulation", ctypes.c_int32)]
+
+
+def fwd_diff(D=1, step=0, tf=False):
+    f = FwdDiff()
+    f.D, f.d_stepsize = D, step
+    f.d_ray_start[:] = [-1, -1, -1]
+    f.d_ray_dir[:] = [-1, -1, -1]
+    if tf:
+        f.has_tf_derivatives = 1
+        f.d_tf = tensor((1, 4, 4), 0, dtype=1)
+    return f
+
+
+@pytest.mark.parametrize("mutate,code,msg", [
+    (lambda f, g: setattr(f, "D", 0), 1, b"number of derivatives must be positive"),
+    (lambda f, g: setattr(f, "d_stepsize", -1), 1, b"at least one derivative must be activated"),
+    (lambda f, g: setattr(f, "d_stepsize", 3), 1, b"d_stepsize = 3 is not a derivative index"),
+    (lambda f, g: f.d_ray_dir.__setitem__(1, 1), 1, b"d_rayDir.y = 1 is not a derivative index"),
+    (lambda f, g: (setattr(f, "has_tf_derivatives", 1), setattr(f, "d_tf", tensor((1, 4, 4), 0, dtype=0))), 1,
+     b"d_tf must be of type torch.int32"),
+    (lambda f, g: (setattr(f, "has_tf_derivatives", 1), setattr(f, "d_tf", tensor((1, 4, 3), 0, dtype=1))), 1,
+     b"d_tf must have 4 entries at dimension 2"),
+    (lambda f, g: setattr(g, "sizes", (ctypes.c_int64 * 6)(1, 6, 8, 2, 4, 0)), 1,
+     b"gradients_out must have 1 entries at dimension 3"),
+])
+def test_forward_gradients_validation(mutate, code, msg):
+    l = lib()
+    i = valid_inputs(0)
+    o = Outputs(tensor((1, 6, 8, 4), 0), tensor((1, 6, 8), 0, dtype=1))
+    f = fwd_diff()
+    g = tensor((1, 6, 8, 1, 4), 0)
+    mutate(f, g)
+    st = l.dvr_render_forward_gradients(ctypes.byref(i), ctypes.byref(f), ctypes.byref(o), ctypes.byref(g), None)
+    assert st == code, (st, l.dvr_last_error())
+    assert msg in l.dvr_last_error()
+
+
+def test_forward_variables_to_gradients_validation():
+    l = lib()
+    fv = tensor((1, 6, 8, 2, 4), 0)
+    ac = tensor((1, 6, 8, 4), 0)
+    a = AdjOut()
+    a.has_volume_derivatives = 1
+    a.adj_volume = tensor((1, 5, 5, 5), 0)
+    f = fwd_diff(D=2)
+    assert l.dvr_forward_variables_to_gradients(ctypes.byref(fv), ctypes.byref(ac), ctypes.byref(f), ctypes.byref(a),
+                                                None) == 1
+    assert b"volume derivatives are not supported" in l.dvr_last_error()
+    a = AdjOut()
+    f = fwd_diff(D=3)
+    assert l.dvr_forward_variables_to_gradients(ctypes.byref(fv), ctypes.byref(ac), ctypes.byref(f), ctypes.byref(a),
+                                                None) == 1
+    assert b"forwardVariables has 2 derivatives" in l.dvr_last_error()
+    f = fwd_diff(D=2)
+    assert l.dvr_forward_variables_to_gradients(ctypes.byref(fv), ctypes.byref(ac), ctypes.byref(f), ctypes.byref(a),
+                                                None) == 2
+    assert b"At least one adjoint variable" in l.dvr_last_error()
+    ac_bad = tensor((1, 6, 7, 4), 0)
+    assert l.dvr_forward_variables_to_gradients(ctypes.byref(fv), ctypes.byref(ac_bad), ctypes.byref(f),
+                                                ctypes.byref(a), None) == 1
+    assert b"gradientOutputColor must have 8 entries at dimension 2" in l.dvr_last_error()
+
+
+def test_compare_to_image_validation():
+    l = lib()
+    col = tensor((1, 6, 8, 4), 0)
+    g = tensor((1, 6, 8, 3, 4), 0)
+    ref = tensor((1, 6, 8, 4), 0)
+    d1, g1 = tensor((1,), 0), tensor((2,), 0)
+    assert l.dvr_compare_to_image(ctypes.byref(col), ctypes.byref(g), ctypes.byref(ref), ctypes.byref(d1),
+                                  ctypes.byref(g1), 1, None) == 1
+    assert b"gradientsOut must have 3 entries at dimension 0" in l.dvr_last_error()
+    d4, g4 = tensor((1, 6, 8, 1), 0), tensor((1, 6, 8, 3), 0)
+    ref_bad = tensor((2, 6, 8, 4), 0)
+    assert l.dvr_compare_to_image(ctypes.byref(col), ctypes.byref(g), ctypes.byref(ref_bad), ctypes.byref(d4),
+                                  ctypes.byref(g4), 0, None) == 1
+    assert b"colorReference must have 1 entries at dimension 0" in l.dvr_last_error()
+    host = tensor((1, 6, 8, 4), -1)
+    assert l.dvr_compare_to_image(ctypes.byref(host), ctypes.byref(g), ctypes.byref(ref), ctypes.byref(d4),
+                                  ctypes.byref(g4), 0, None) == 3

@@ -1,0 +1,298 @@
+"""Forward-mode differentiation on the GPU (renderer/renderer_kernels.cuh:135-308,
+:504-598; renderer/renderer.cpp:419-658) against the FP64 oracle.
+
+* colour and termination index of render_forward_gradients equal the tiled
+  forward bit for bit (same march, renderer.h:166-171 states this);
+* the Jacobian columns contracted with a random colour adjoint equal the
+  oracle's adjoint gradients (transposition: the reference's adjoint, like its
+  forward mode, does not trace tmin): normwise relative error < 1e-4;
+* at the TF clamps the derivative follows the reference adjoint's convention
+  (renderer_tf.cuh:103, :337-357), so the transposition holds everywhere;
+* every column (step size, ray start / direction, each TF entry) equals FP64
+  central differences of the oracle forward with the GPU's sample count and
+  tmin frozen, per pixel, on pixels where the difference quotient is smooth
+  (same quotient at eps and eps/4): normwise relative error < 1e-4;
+* forward_variables_to_gradients equals the explicit float64 contraction and
+  accumulates; compare_to_image equals the float64 formula;
+* diffdvr.Renderer(gradient_method="forward") gives the adjoint's gradients.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import oracle, camera_scene
+from gpu_helpers import render, gpu_inputs, t32, dev, norm_err, batch
+from diffdvr_amd import pyrenderer as pr
+from diffdvr_amd import diffdvr
+
+pytestmark = pytest.mark.gpu
+
+
+def settings(s, step=True, camera=True, tf=True, tf_batch=None):
+    """ForwardDifferencesSettings tracing step, ray start/dir and every TF entry."""
+    fd = pr.ForwardDifferencesSettings()
+    cols = {}
+    D = 0
+    if step:
+        fd.d_stepsize = D
+        cols["step"] = D
+        D += 1
+    if camera:
+        fd.d_rayStart = pr.int3(D, D + 1, D + 2)
+        fd.d_rayDir = pr.int3(D + 3, D + 4, D + 5)
+        cols["ray_start"], cols["ray_dir"] = [D, D + 1, D + 2], [D + 3, D + 4, D + 5]
+        D += 6
+    if tf:
+        Bt, R, C = s["tf"].shape
+        Bt = tf_batch if tf_batch is not None else Bt
+        idx = torch.arange(D, D + R * C, dtype=torch.int32).reshape(1, R, C).expand(Bt, R, C).contiguous()
+        fd.d_tf = idx.to(dev())
+        fd.has_tf_derivatives = True
+        cols["tf"] = idx[0].numpy()
+        D += R * C
+    fd.D = D
+    return fd, cols
+
+
+def render_fg(s, fd):
+    B, H, W = batch(s), int(s["H"]), int(s["W"])
+    inp = gpu_inputs(s)
+    color = torch.full((B, H, W, 4), float("nan"), device=dev())
+    term = torch.full((B, H, W), -7, dtype=torch.int32, device=dev())
+    g = torch.full((B, H, W, fd.D, 4), float("nan"), device=dev())
+    pr.Renderer.render_forward_gradients(inp, fd, pr.RendererOutputs(color, term), g)
+    torch.cuda.synchronize()
+    return inp, color, term, g
+
+
+def contract(g, adj, cols, tf_shape):
+    """float64 contraction of the Jacobian G (B,H,W,D,4) with adj (B,H,W,4)."""
+    G = g.double().cpu().numpy()
+    a = np.asarray(adj, dtype=np.float64)
+    per = np.einsum("bhwdc,bhwc->bhwd", G, a)
+    out = {}
+    if "step" in cols:
+        out["step"] = per[..., cols["step"]]
+    if "ray_start" in cols:
+        out["ray_start"] = per[..., cols["ray_start"]]
+        out["ray_dir"] = per[..., cols["ray_dir"]]
+    if "tf" in cols:
+        tot = per.sum(axis=(1, 2))  # (B, D)
+        Bt = tf_shape[0]
+        tf = np.zeros(tf_shape)
+        for b in range(tot.shape[0]):
+            tf[b if Bt > 1 else 0] += tot[b][cols["tf"]]
+        out["tf"] = tf
+    return out
+
+
+def compare(got, want, tol, label=""):
+    errs = {}
+    for k, v in want.items():
+        g = got[k]
+        if np.linalg.norm(v) == 0:
+            assert np.abs(g).max() < 1e-6, (label, k, np.abs(g).max())
+            continue
+        errs[k] = norm_err(g, v)
+        assert errs[k] < tol, (label, k, errs[k])
+    return errs
+
+
+@pytest.mark.parametrize("filter", [0, 1])
+@pytest.mark.parametrize("tf_mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("blend", [0, 1])
+def test_forward_gradients_transpose_adjoint(filter, tf_mode, blend):
+    rng = np.random.default_rng(6000 + 100 * filter + 10 * tf_mode + blend)
+    s = camera_scene(rng, 19, 17, B=2, filter=filter, tf_mode=tf_mode, blend=blend, camera=0, smooth=True,
+                     X=16, Y=14, Z=18, step=0.025)
+    if blend == 1:
+        s["tf"][..., 0 if tf_mode == 0 else 3] *= 0.5
+    fd, cols = settings(s)
+    _, color, term, g = render_fg(s, fd)
+    _, c_ref, t_ref = render(s, path="tiled")
+    assert torch.equal(term, t_ref)
+    assert torch.equal(color, c_ref)
+    assert torch.isfinite(g).all()
+    t_gpu = term.cpu().numpy()
+    c_ora, _ = oracle.forward(s, frozen_term=t_gpu)
+    grads = ("tf", "camera", "step")
+    for trial in range(2):
+        if trial == 0:
+            adj = rng.standard_normal(color.shape)
+        else:  # one channel at a time still matches (no channel mixed up)
+            adj = np.zeros(color.shape)
+            adj[..., int(rng.integers(4))] = 1.0
+        want = oracle.adjoint(s, c_ora, t_gpu, adj, grads=grads, method=oracle.EXACT_REPLAY)
+        got = contract(g, adj, cols, s["tf"].shape)
+        errs = compare(got, want, 1e-4, (filter, tf_mode, blend, trial))
+        print(filter, tf_mode, blend, trial, {k: f"{v:.1e}" for k, v in errs.items()}, "D =", fd.D)
+
+
+@pytest.mark.parametrize("tf_mode", [1, 2, 3])
+def test_forward_gradients_central_differences(tf_mode):
+    rng = np.random.default_rng(6100 + tf_mode)
+    s = camera_scene(rng, 13, 11, B=1, filter=1, tf_mode=tf_mode, blend=0, camera=0, smooth=True,
+                     X=14, Y=12, Z=16, step=0.03)
+    if tf_mode == 2:
+        # end control points outside the density range: the clamp (whose
+        # derivative follows the adjoint's convention, not a one-sided limit)
+        # is never active, and no density sits on a control point
+        s["tf"][0, 0, 4] = float(np.float32(-0.05))
+        s["tf"][0, -1, 4] = float(np.float32(1.05))
+    fd, cols = settings(s)
+    _, color, term, g = render_fg(s, fd)
+    G = g.double().cpu().numpy()[0]  # H, W, D, 4
+    t_gpu = term.cpu().numpy()
+    _, _, t0 = oracle.forward(s, frozen_term=t_gpu, return_t0=True)
+
+    def column(key, idx, e):
+        sp = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in s.items()}
+        sm = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in s.items()}
+        sp[key][idx] += e
+        sm[key][idx] -= e
+        cp, _ = oracle.forward(sp, frozen_term=t_gpu, frozen_t0=t0)
+        cm, _ = oracle.forward(sm, frozen_term=t_gpu, frozen_t0=t0)
+        return ((cp - cm) / (2 * e))[0]
+
+    probes = [("step", (Ellipsis,), cols["step"])]
+    for c in range(3):
+        probes.append(("ray_start", (Ellipsis, c), cols["ray_start"][c]))
+        probes.append(("ray_dir", (Ellipsis, c), cols["ray_dir"][c]))
+    R, C = s["tf"].shape[1:]
+    for r in range(R):
+        for c in range(C):
+            probes.append(("tf", (0, r, c), int(cols["tf"][r, c])))
+    worst = 0.0
+    smooth_frac = []
+    for key, idx, d in probes:
+        e = 1e-6
+        f1, f2 = column(key, idx, e), column(key, idx, e / 4)
+        scale = np.abs(f1).max() + 1e-30
+        ok = (np.abs(f1 - f2) <= 1e-7 * scale).all(axis=-1)
+        smooth_frac.append(ok.mean())
+        want = f1[ok]
+        got = G[:, :, d, :][ok]
+        if np.linalg.norm(want) < 1e-9:
+            assert np.abs(got).max() < 1e-5, (key, idx)
+            continue
+        err = norm_err(got, want)
+        worst = max(worst, err)
+        assert err < 1e-4, (key, idx, err)
+    assert np.mean(smooth_frac) > 0.8, smooth_frac
+    print(f"tf_mode {tf_mode}: {len(probes)} columns, worst normwise error vs FP64 central differences {worst:.2e}")
+
+
+def test_reference_frame_broadcast_tf():
+    """ReferenceFrame camera, 3 views sharing one TF (batch 1): the TF adjoint sums over views."""
+    rng = np.random.default_rng(6200)
+    s = camera_scene(rng, 24, 20, B=3, filter=1, tf_mode=2, camera=2, smooth=True, X=16, Y=16, Z=16, step=0.02)
+    fd, cols = settings(s, camera=False)
+    _, color, term, g = render_fg(s, fd)
+    _, c_ref, t_ref = render(s, path="tiled")
+    assert torch.equal(term, t_ref) and torch.equal(color, c_ref)
+    t_gpu = term.cpu().numpy()
+    c_ora, _ = oracle.forward(s, frozen_term=t_gpu)
+    adj = rng.standard_normal(color.shape)
+    want = oracle.adjoint(s, c_ora, t_gpu, adj, grads=("tf", "step"), method=oracle.EXACT_REPLAY)
+    # through the library's contraction, not numpy
+    a = pr.AdjointOutputs()
+    a.has_tf_derivatives = True
+    a.adj_tf = torch.zeros(s["tf"].shape, device=dev())
+    a.has_stepsize_derivatives = True
+    a.adj_stepsize = torch.zeros(color.shape[:3], device=dev())
+    pr.Renderer.forward_variables_to_gradients(g, t32(adj), fd, a)
+    torch.cuda.synchronize()
+    got = {"tf": a.adj_tf.double().cpu().numpy(), "step": a.adj_stepsize.double().cpu().numpy()}
+    errs = compare(got, want, 1e-4, "refframe")
+    print("ReferenceFrame, broadcast TF:", {k: f"{v:.1e}" for k, v in errs.items()})
+
+
+def test_forward_variables_to_gradients_contraction():
+    rng = np.random.default_rng(6300)
+    s = camera_scene(rng, 21, 18, B=2, filter=1, tf_mode=1, camera=0, smooth=True, X=12, Y=12, Z=12)
+    fd, cols = settings(s)
+    _, color, term, g = render_fg(s, fd)
+    adj = t32(rng.standard_normal(color.shape))
+    want = contract(g, adj.double().cpu().numpy(), cols, s["tf"].shape)
+    B, H, W = color.shape[:3]
+    a = pr.AdjointOutputs()
+    a.has_tf_derivatives = True
+    a.adj_tf = torch.zeros(s["tf"].shape, device=dev())
+    a.has_stepsize_derivatives = True
+    a.adj_stepsize = torch.zeros(B, H, W, device=dev())
+    a.has_camera_derivatives = True
+    a.adj_camera_ray_start = torch.zeros(B, H, W, 3, device=dev())
+    a.adj_camera_ray_dir = torch.zeros(B, H, W, 3, device=dev())
+    for rep in (1, 2):  # accumulates
+        pr.Renderer.forward_variables_to_gradients(g, adj, fd, a)
+        torch.cuda.synchronize()
+        got = {"tf": a.adj_tf, "step": a.adj_stepsize, "ray_start": a.adj_camera_ray_start,
+               "ray_dir": a.adj_camera_ray_dir}
+        for k, v in got.items():
+            err = norm_err(v.double().cpu().numpy(), rep * want[k])
+            assert err < 1e-5, (k, rep, err)
+    # broadcast outputs: batch-1 camera adjoint and a 1x1x1 step adjoint receive the sums
+    b = pr.AdjointOutputs()
+    b.has_stepsize_derivatives = True
+    b.adj_stepsize = torch.zeros(1, 1, 1, device=dev())
+    b.has_camera_derivatives = True
+    b.adj_camera_ray_start = torch.zeros(1, H, W, 3, device=dev())
+    b.adj_camera_ray_dir = torch.zeros(1, H, W, 3, device=dev())
+    pr.Renderer.forward_variables_to_gradients(g, adj, fd, b)
+    torch.cuda.synchronize()
+    assert float(b.adj_stepsize) == pytest.approx(want["step"].sum(), rel=1e-4, abs=1e-6)
+    assert norm_err(b.adj_camera_ray_start[0].double().cpu().numpy(), want["ray_start"].sum(0)) < 1e-5
+    assert norm_err(b.adj_camera_ray_dir[0].double().cpu().numpy(), want["ray_dir"].sum(0)) < 1e-5
+
+
+@pytest.mark.parametrize("reduce", [False, True])
+def test_compare_to_image(reduce):
+    rng = np.random.default_rng(6400 + int(reduce))
+    B, H, W, D = 2, 9, 13, 5
+    col = rng.uniform(0, 1, (B, H, W, 4))
+    ref = rng.uniform(0, 1, (B, H, W, 4))
+    ref[0, 0, 0] = col[0, 0, 0]  # L = 0: derivative 0
+    gin = rng.standard_normal((B, H, W, D, 4))
+    col, ref, gin = [np.asarray(x, dtype=np.float32).astype(np.float64) for x in (col, ref, gin)]
+    diff = col - ref
+    L = np.linalg.norm(diff, axis=-1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        dL = np.where(L[..., None] > 0, np.einsum("bhwc,bhwdc->bhwd", diff, gin) / L[..., None], 0.0)
+    if reduce:
+        dout = torch.full((1,), 7.0, device=dev())
+        gout = torch.full((D,), 7.0, device=dev())
+    else:
+        dout = torch.full((B, H, W, 1), 7.0, device=dev())
+        gout = torch.full((B, H, W, D), 7.0, device=dev())
+    pr.Renderer.compare_to_image(t32(col), t32(gin), t32(ref), dout, gout, reduce)
+    torch.cuda.synchronize()
+    if reduce:
+        assert float(dout) == pytest.approx(L.mean(), rel=1e-5)
+        assert norm_err(gout.double().cpu().numpy(), dL.mean(axis=(0, 1, 2))) < 1e-5
+    else:
+        assert norm_err(dout[..., 0].double().cpu().numpy(), L) < 1e-6
+        assert norm_err(gout.double().cpu().numpy(), dL) < 1e-6
+        assert float(gout[0, 0, 0].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("delayed", [False, True])
+def test_diffdvr_forward_method_matches_adjoint(delayed):
+    rng = np.random.default_rng(6500 + int(delayed))
+    s = camera_scene(rng, 24, 20, B=1, filter=1, tf_mode=1, camera=2, smooth=True, X=16, Y=16, Z=16, step=0.02)
+    inp = gpu_inputs(s)
+    viewport = t32(s["cam_matrix"])
+    target = torch.rand(1, 20, 24, 4, device=dev())
+    grads = {}
+    for method in ("adjoint", "forward"):
+        r = diffdvr.Renderer(inp.clone(), optimize_camera=True, optimize_tf=True, gradient_method=method,
+                             forward_delayed_gradients=delayed)
+        tf = t32(s["tf"]).requires_grad_(True)
+        vp = viewport.clone().requires_grad_(True)
+        img = r(camera=vp, fov_y_radians=float(s["fov_y"]), tf=tf)
+        loss = torch.nn.functional.mse_loss(img, target)
+        loss.backward()
+        torch.cuda.synchronize()
+        grads[method] = (img.detach(), tf.grad.double().cpu().numpy(), vp.grad.double().cpu().numpy())
+    assert torch.equal(grads["adjoint"][0], grads["forward"][0])
+    assert norm_err(grads["forward"][1], grads["adjoint"][1]) < 1e-4
+    assert norm_err(grads["forward"][2], grads["adjoint"][2]) < 1e-4
