@@ -725,6 +725,223 @@ inline void render_adjoint(const Inputs<T>& in, const T* color, const int* term,
 }
 
 // ---------------------------------------------------------------------------
+// Forward-mode differentiation (renderer_kernels.cuh:135-308,
+// renderer_tf.cuh:122-145,256-287,437-485,522-560): every traced quantity is
+// a dual number carrying D derivatives, as the reference's cudAD
+// fvar<real_t, D>.  Traced inputs: step size, the evaluated ray start and
+// direction, TF entries (index table d_tf, -1 = not traced).  tmin is not
+// traced (the reference casts the ray to real for the slab test).
+//
+// cudAD is not available, so its clamp convention is taken from the
+// reference's adjoint, which the reference's consistency tests hold the
+// forward mode to (unittests/testConsistencyTF.cpp:183-204):
+//   identity TF : density derivative passes strictly inside (0,1) only
+//                 (renderer_tf.cuh:103);
+//   linear TF   : clamp(d, p0, p1) keeps the derivative of d, the fraction is
+//                 differentiated with the clamped value (renderer_tf.cuh:337-357);
+//   alpha blend : min(1, x) passes the derivative of x iff x < 1
+//                 (renderer_blending.cuh:118-125 with the adjoint :147-159).
+// ---------------------------------------------------------------------------
+template <class T> struct Dual {
+    T v;
+    std::vector<T> d;
+};
+template <class T> inline Dual<T> dconst(T v, int D) { return Dual<T>{v, std::vector<T>((size_t)D, T(0))}; }
+template <class T> inline Dual<T> dinput(T v, int idx, int D)
+{
+    Dual<T> r = dconst(v, D);
+    if (idx >= 0) r.d[(size_t)idx] = T(1);
+    return r;
+}
+template <class T> inline Dual<T> operator+(const Dual<T>& a, const Dual<T>& b)
+{
+    Dual<T> r{a.v + b.v, a.d};
+    for (size_t k = 0; k < r.d.size(); ++k) r.d[k] += b.d[k];
+    return r;
+}
+template <class T> inline Dual<T> operator-(const Dual<T>& a, const Dual<T>& b)
+{
+    Dual<T> r{a.v - b.v, a.d};
+    for (size_t k = 0; k < r.d.size(); ++k) r.d[k] -= b.d[k];
+    return r;
+}
+template <class T> inline Dual<T> operator*(const Dual<T>& a, const Dual<T>& b)
+{
+    Dual<T> r{a.v * b.v, a.d};
+    for (size_t k = 0; k < r.d.size(); ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+template <class T> inline Dual<T> operator/(const Dual<T>& a, const Dual<T>& b)
+{
+    Dual<T> r{a.v / b.v, a.d};
+    for (size_t k = 0; k < r.d.size(); ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+    return r;
+}
+template <class T> inline Dual<T> operator*(T s, const Dual<T>& a)
+{
+    Dual<T> r{s * a.v, a.d};
+    for (auto& x : r.d) x *= s;
+    return r;
+}
+template <class T> inline Dual<T> operator-(const Dual<T>& a, T s) { return Dual<T>{a.v - s, a.d}; }
+template <class T> inline Dual<T> operator+(const Dual<T>& a, T s) { return Dual<T>{a.v + s, a.d}; }
+template <class T> inline Dual<T> operator-(T s, const Dual<T>& a) { return T(-1) * a + s; }
+template <class T> inline Dual<T> operator/(const Dual<T>& a, T s)
+{
+    Dual<T> r{a.v / s, a.d};
+    for (auto& x : r.d) x /= s;
+    return r;
+}
+template <class T> inline Dual<T> dexp(const Dual<T>& a)
+{
+    const T e = std::exp(a.v);
+    Dual<T> r{e, a.d};
+    for (auto& x : r.d) x *= e;
+    return r;
+}
+template <class T> inline Dual<T> dlerp(const Dual<T>& a, const Dual<T>& b, const Dual<T>& t) { return a + t * (b - a); }
+
+struct FwdDiffSettings {
+    int D;
+    int d_step;
+    int d_start[3], d_dir[3];
+    const int* d_tf;  // d_tf_b x R x C, or null
+    int d_tf_b;
+};
+
+template <class T>
+inline Dual<T> tf_entry(const Inputs<T>& in, const FwdDiffSettings& fd, int b, int r, int c)
+{
+    const int idx = fd.d_tf ? fd.d_tf[((size_t)bidx(fd.d_tf_b, b) * in.R + r) * in.C + c] : -1;
+    return dinput(tf_row(in, b, r)[c], idx, fd.D);
+}
+
+template <class T>
+inline void tf_eval_dual(const Inputs<T>& in, const FwdDiffSettings& fd, int b, const Dual<T>& density,
+                         Dual<T> out[4])
+{
+    const int D = fd.D;
+    switch (in.tf_mode) {
+    case TFIdentity: {
+        Dual<T> dc{fclamp(density.v, T(0), T(1)), density.d};
+        if (!(density.v > T(0) && density.v < T(1))) dc.d.assign((size_t)D, T(0));
+        const Dual<T> sa = tf_entry(in, fd, b, 0, 0), sc = tf_entry(in, fd, b, 0, 1);
+        out[0] = dc * sc; out[1] = dc * sc; out[2] = dc * sc; out[3] = dc * sa;
+        return;
+    }
+    case TFTexture: {
+        const int R = in.R;
+        const Dual<T> d = T(R) * density - T(0.5);
+        const int di = (int)std::floor(d.v);
+        const Dual<T> df = d - T(di);
+        const int r0 = iclamp(di, 0, R - 1), r1 = iclamp(di + 1, 0, R - 1);
+        for (int c = 0; c < 4; ++c) out[c] = dlerp(tf_entry(in, fd, b, r0, c), tf_entry(in, fd, b, r1, c), df);
+        return;
+    }
+    case TFLinear: {
+        const int R = in.R;
+        int i;
+        for (i = 0; i < R - 2; ++i)
+            if (tf_row(in, b, i + 1)[4] > density.v) break;
+        const Dual<T> p0 = tf_entry(in, fd, b, i, 4), p1 = tf_entry(in, fd, b, i + 1, 4);
+        const Dual<T> d2{fclamp(density.v, p0.v, p1.v), density.d};
+        const Dual<T> f = (d2 - p0) / (p1 - p0);
+        for (int c = 0; c < 4; ++c) out[c] = dlerp(tf_entry(in, fd, b, i, c), tf_entry(in, fd, b, i + 1, c), f);
+        return;
+    }
+    default: {
+        for (int c = 0; c < 4; ++c) out[c] = dconst(T(0), D);
+        for (int i = 0; i < in.R; ++i) {
+            const Dual<T> mu = tf_entry(in, fd, b, i, 4), s = tf_entry(in, fd, b, i, 5);
+            const Dual<T> dm = density - mu;
+            const Dual<T> n = dexp(T(-1) * (dm * dm) / (T(2) * (s * s)));
+            for (int c = 0; c < 4; ++c) out[c] = out[c] + tf_entry(in, fd, b, i, c) * n;
+        }
+        return;
+    }
+    }
+}
+
+// One ray; grad receives D x 4 derivatives of the RGBA output.
+template <class T>
+inline void march_forward_gradients(const Inputs<T>& in, const FwdDiffSettings& fd, int x, int y, int b,
+                                    int frozen_n, const T* frozen_t0, V4<T>& color, int& term, T* grad)
+{
+    const int D = fd.D;
+    Ray<T> r = setup_ray(in, x, y, b);
+    if (frozen_t0) r.t0 = *frozen_t0;
+    const T sv[3] = {r.start.x, r.start.y, r.start.z}, dv[3] = {r.dir.x, r.dir.y, r.dir.z};
+    const T bm[3] = {r.bmin.x, r.bmin.y, r.bmin.z}, vx[3] = {r.voxel.x, r.voxel.y, r.voxel.z};
+    std::vector<Dual<T>> start, dir;
+    for (int c = 0; c < 3; ++c) {
+        start.push_back(dinput(sv[c], fd.d_start[c], D));
+        dir.push_back(dinput(dv[c], fd.d_dir[c], D));
+    }
+    const Dual<T> step = dinput(r.step, fd.d_step, D);
+    Dual<T> acc[4] = {dconst(T(0), D), dconst(T(0), D), dconst(T(0), D), dconst(T(0), D)};
+    const int vb = bidx(in.vol_b, b);
+    int i;
+    for (i = 0; i < (1 << 24); ++i) {
+        const Dual<T> t = T(i) * step + r.t0;
+        if (frozen_n >= 0) {
+            if (i >= frozen_n) break;
+        } else {
+            if (t.v > r.t1) break;
+            if (acc[3].v > in.early_out) break;
+        }
+        Dual<T> p[3];
+        for (int c = 0; c < 3; ++c) p[c] = ((start[c] + t * dir[c]) - bm[c]) / vx[c];
+        Dual<T> density;
+        if (in.filter == FilterNearest) {
+            density = dconst(fetch_density(in, b, v3(p[0].v, p[1].v, p[2].v)), D);
+        } else {
+            TriCell<T> cell;
+            trilinear_cell(in, vb, v3(p[0].v, p[1].v, p[2].v), cell);
+            const Dual<T> fx = p[0] - T(cell.ix), fy = p[1] - T(cell.iy), fz = p[2] - T(cell.iz);
+            Dual<T> cd[8];
+            for (int k = 0; k < 8; ++k) cd[k] = dconst(cell.d[k], D);
+            density = dlerp(dlerp(dlerp(cd[0], cd[4], fx), dlerp(cd[2], cd[6], fx), fy),
+                            dlerp(dlerp(cd[1], cd[5], fx), dlerp(cd[3], cd[7], fx), fy), fz);
+        }
+        Dual<T> s[4];
+        tf_eval_dual(in, fd, b, density, s);
+        // blending, reference form acc + (1 - acc.a) * alpha * (rgb, 1)
+        Dual<T> alpha;
+        if (in.blend == BlendBeerLambert) {
+            alpha = T(1) - dexp(T(-1) * (s[3] * step));
+        } else {
+            const Dual<T> xa = s[3] * step;
+            alpha = xa.v < T(1) ? xa : dconst(T(1), D);
+        }
+        const Dual<T> w = (T(1) - acc[3]) * alpha;
+        for (int c = 0; c < 3; ++c) acc[c] = acc[c] + w * s[c];
+        acc[3] = acc[3] + w;
+    }
+    color = v4(acc[0].v, acc[1].v, acc[2].v, acc[3].v);
+    term = i;
+    for (int k = 0; k < D; ++k)
+        for (int c = 0; c < 4; ++c) grad[4 * k + c] = acc[c].d[(size_t)k];
+}
+
+template <class T>
+inline void render_forward_gradients(const Inputs<T>& in, const FwdDiffSettings& fd, T* color, int* term, T* grad,
+                                     const int* frozen_term, const T* frozen_t0)
+{
+    const long long count = (long long)in.B * in.H * in.W;
+#pragma omp parallel for schedule(guided)
+    for (long long i = 0; i < count; ++i) {
+        const int b = (int)(i / ((long long)in.W * in.H));
+        const int y = (int)((i - (long long)b * in.W * in.H) / in.W);
+        const int x = (int)(i - (long long)in.W * (y + (long long)in.H * b));
+        V4<T> c; int n;
+        march_forward_gradients(in, fd, x, y, b, frozen_term ? frozen_term[i] : -1,
+                                frozen_t0 ? frozen_t0 + i : (const T*)nullptr, c, n, grad + (size_t)i * fd.D * 4);
+        color[4 * i + 0] = c.x; color[4 * i + 1] = c.y; color[4 * i + 2] = c.z; color[4 * i + 3] = c.w;
+        term[i] = n;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Kink census (test support).  Counts the samples of a ray (sample count n
 // frozen) whose FP64 density lies within eps * (largest corner magnitude of
 // its cell, the scale of FP32 interpolation error) of a point where the

@@ -83,6 +83,16 @@ template <class T> oracle::Inputs<T> tf_only(int mode, const T* tf, int R, int C
     {                                                                                                          \
         oracle::kink_census<T>(convert<T>(in), term, eps, eps_pos, out);                                       \
     }                                                                                                          \
+    /* forward-mode derivatives: grad is B*H*W*D*4; d_start/d_dir hold 3 indices; d_tf may be null */          \
+    extern "C" void oracle_render_forward_gradients_##SUF(const oracle_inputs* in, int D, int d_step,            \
+                                                            const int* d_start, const int* d_dir,                \
+                                                            const int* d_tf, int d_tf_b, T* color, int* term,    \
+                                                            T* grad, const int* frozen_term, const T* frozen_t0)  \
+    {                                                                                                          \
+        oracle::FwdDiffSettings fd{D, d_step, {d_start[0], d_start[1], d_start[2]},                            \
+                                   {d_dir[0], d_dir[1], d_dir[2]}, d_tf, d_tf_b};                              \
+        oracle::render_forward_gradients<T>(convert<T>(in), fd, color, term, grad, frozen_term, frozen_t0);    \
+    }                                                                                                          \
     extern "C" void oracle_render_adjoint_##SUF(const oracle_inputs* in, const T* color, const int* term,       \
                                                   const T* adj_color, const oracle_adjoint_outputs* out,          \
                                                   int method, const T* frozen_t0)                                 \

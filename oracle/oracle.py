@@ -150,6 +150,27 @@ def forward(scene, dtype=np.float64, frozen_term=None, frozen_t0=None, return_t0
     return (color, term, t0) if return_t0 else (color, term)
 
 
+def forward_gradients(scene, D, d_step=-1, d_start=(-1, -1, -1), d_dir=(-1, -1, -1), d_tf=None, dtype=np.float64,
+                      frozen_term=None, frozen_t0=None):
+    """Oracle forward-mode derivatives -> color (B,H,W,4), term (B,H,W), grad (B,H,W,D,4).
+
+    d_tf: int array (1 or B, R, C) of derivative indices (-1 = not traced) or None."""
+    p = _Prepared(scene, dtype)
+    B, H, W = p.B, p.H, p.W
+    color = np.zeros((B, H, W, 4), dtype=p.dtype)
+    term = np.zeros((B, H, W), dtype=np.int32)
+    grad = np.zeros((B, H, W, int(D), 4), dtype=p.dtype)
+    ds = np.ascontiguousarray(d_start, dtype=np.int32)
+    dd = np.ascontiguousarray(d_dir, dtype=np.int32)
+    dt = None if d_tf is None else np.ascontiguousarray(d_tf, dtype=np.int32)
+    ft = None if frozen_term is None else np.ascontiguousarray(frozen_term, dtype=np.int32)
+    f0 = None if frozen_t0 is None else np.ascontiguousarray(frozen_t0, dtype=p.dtype)
+    getattr(lib(), f"oracle_render_forward_gradients_{_suffix(dtype)}")(
+        ctypes.byref(p.s), int(D), int(d_step), _ptr(ds), _ptr(dd), _ptr(dt), 0 if dt is None else dt.shape[0],
+        _ptr(color), _ptr(term), _ptr(grad), _ptr(ft), _ptr(f0))
+    return color, term, grad
+
+
 def adjoint(scene, color, term, adj_color, grads=("tf",), dtype=np.float64, method=INVERSION, frozen_t0=None,
             adj_volume_batch=None, adj_tf_batch=None, adj_cam_batch=None, adj_step_shape=None):
     """Oracle adjoint -> dict with the requested gradients (accumulated from zero)."""

@@ -87,3 +87,33 @@ def camera_scene(rng, W, H, B=1, filter=1, tf_mode=1, blend=0, camera=0, X=12, Y
         m = np.array(pr.Camera.compute_matrix(pr.real3(-1.2, -0.6, 0.8), pr.real3(0, 0, 0), pr.real3(0, 0, 1), 45, W, H))
         s["cam_matrix"] = to_f32(m[None])
     return s
+
+
+def marschner_lobb(n=32, fM=6.0, alpha=0.25):
+    """renderer/volume_implicit.cpp:28-42 sampled as renderer/volume.cpp:369-387."""
+    scale = np.float32(2.0) / np.float32(n - 1)
+    c = (np.float32(-1.0) + np.arange(n, dtype=np.float32) * scale).astype(np.float64)
+    x, y, z = np.meshgrid(c, c, c, indexing="ij")
+    r = np.sqrt(x * x + y * y)
+    pr_ = np.cos(2 * math.pi * fM * np.cos(math.pi * r / 2))
+    v = ((1 - np.sin(math.pi * z / 2)) + alpha * (1 + pr_)) / (2 * (1 + alpha))
+    return v.astype(np.float32).astype(np.float64)[None]
+
+
+def consistency_tfs(rng):
+    """The TF configurations of testConsistencyTF.cpp:71-101 (uniform random entries; the linear TF's
+    inner positions are i/(R-1) in integer arithmetic, i.e. 0)."""
+    out = []
+    for mode, R, C in ((0, 1, 2), (1, 2, 4), (1, 4, 4), (1, 8, 4), (2, 4, 5)):
+        tf = to_f32(rng.uniform(0, 1, (1, R, C)))
+        if mode == 2:
+            tf[0, :, 4] = 0.0
+            tf[0, -1, 4] = 1.0
+        out.append((mode, tf))
+    return out
+
+
+def sym_err(a, b):
+    """Symmetric error of testConsistencyTF.cpp:15-41."""
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return float((np.abs(a - b) / np.maximum(1.0, np.abs(a) + np.abs(b))).max())

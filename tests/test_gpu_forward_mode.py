@@ -20,7 +20,9 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import oracle, camera_scene
+import math
+
+from helpers import oracle, camera_scene, marschner_lobb, consistency_tfs, sym_err, to_f32
 from gpu_helpers import render, gpu_inputs, t32, dev, norm_err, batch
 from diffdvr_amd import pyrenderer as pr
 from diffdvr_amd import diffdvr
@@ -115,6 +117,18 @@ def test_forward_gradients_transpose_adjoint(filter, tf_mode, blend):
     assert torch.isfinite(g).all()
     t_gpu = term.cpu().numpy()
     c_ora, _ = oracle.forward(s, frozen_term=t_gpu)
+    # element-wise: the GPU Jacobian against the oracle's dual-number Jacobian (FP64)
+    _, _, G_ora = oracle.forward_gradients(s, fd.D, d_step=cols["step"], d_start=cols["ray_start"],
+                                           d_dir=cols["ray_dir"], d_tf=cols["tf"][None], frozen_term=t_gpu)
+    G = g.double().cpu().numpy()
+    for key in ("step", "ray_start", "ray_dir", "tf"):
+        idx = np.atleast_1d(np.asarray(cols[key])).ravel()
+        want_j = G_ora[..., idx, :]
+        if np.linalg.norm(want_j) == 0:
+            assert np.abs(G[..., idx, :]).max() < 1e-6, key
+            continue
+        ej = norm_err(G[..., idx, :], want_j)
+        assert ej < 1e-4, (key, ej)
     grads = ("tf", "camera", "step")
     for trial in range(2):
         if trial == 0:
@@ -296,3 +310,98 @@ def test_diffdvr_forward_method_matches_adjoint(delayed):
     assert torch.equal(grads["adjoint"][0], grads["forward"][0])
     assert norm_err(grads["forward"][1], grads["adjoint"][1]) < 1e-4
     assert norm_err(grads["forward"][2], grads["adjoint"][2]) < 1e-4
+
+
+def _consistency_settings(R, C):
+    fd = pr.ForwardDifferencesSettings()
+    fd.D = R * C
+    fd.d_tf = torch.arange(R * C, dtype=torch.int32, device=dev()).reshape(1, R, C)
+    fd.has_tf_derivatives = True
+    return fd
+
+
+def test_reference_consistency_tf_gpu():
+    """unittests/testConsistencyTF.cpp:43-226 on the GPU: Marschner-Lobb 32^3, 128^2 pixels, InverseViewMatrix
+    camera; forward-mode TF gradient (render_forward_gradients + forward_variables_to_gradients) vs the adjoint
+    (inversion immediate and delayed, checkpointed), symmetric error; plus both vs the FP64 oracle."""
+    rng = np.random.default_rng(7400)
+    W = H = 128
+    m = np.array(pr.Camera.compute_matrix(pr.real3(-1, -0.5, 0), pr.real3(0, 0, 0), pr.real3(0, 0, 1), 45, W, H,
+                                          0.1, 10.0))
+    base = dict(W=W, H=H, filter=1, camera=1, blend=0, early_out=1.0 - 1e-5, fov_y=0.0,
+                volume=marschner_lobb(32), box_min=np.full((1, 3), -0.5), box_size=np.ones((1, 3)),
+                step=np.full((1, 1, 1), float(np.float32(0.5 / 64))), cam_matrix=to_f32(m[None]))
+    worst = 0.0
+    for mode, tf in consistency_tfs(rng):
+        s = dict(base, tf_mode=mode, tf=tf)
+        R, C = tf.shape[1:]
+        fd = _consistency_settings(R, C)
+        inp, color, term, g = render_fg(s, fd)
+        adj = t32(rng.standard_normal(color.shape))
+        a = pr.AdjointOutputs()
+        a.has_tf_derivatives = True
+        a.adj_tf = torch.zeros(1, R, C, device=dev())
+        pr.Renderer.forward_variables_to_gradients(g, adj, fd, a)
+        fwd = a.adj_tf.double().cpu().numpy()
+        results = {}
+        for label, kw, delayed in (("immediate", dict(method="inversion", path="plain"), False),
+                                   ("delayed", dict(method="inversion", path="plain"), True),
+                                   ("checkpoint", dict(method="checkpoint"), True)):
+            b = pr.AdjointOutputs()
+            b.has_tf_derivatives = True
+            b.tf_delayed_accumulation = delayed
+            b.adj_tf = torch.zeros(1, R, C, device=dev())
+            pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), adj, b, **kw)
+            results[label] = b.adj_tf.double().cpu().numpy()
+        torch.cuda.synchronize()
+        c_ora, _ = oracle.forward(s, frozen_term=term.cpu().numpy())
+        want = oracle.adjoint(s, c_ora, term.cpu().numpy(), adj.double().cpu().numpy(), grads=("tf",),
+                              method=oracle.EXACT_REPLAY)["tf"]
+        errs = {k: sym_err(fwd, v) for k, v in results.items()}
+        errs["oracle"] = sym_err(fwd, want)
+        print(f"TF mode {mode} R={R}:", {k: f"{v:.1e}" for k, v in errs.items()})
+        # the reference's EPS against the FP64 oracle and the checkpointed adjoint; the inversion
+        # adjoint adds its own FP32 error (~1/(1-A) amplification, atomics in thread order)
+        assert errs["oracle"] <= 1e-5 and errs["checkpoint"] <= 1e-5, errs
+        worst = max(worst, max(errs.values()))
+    assert worst <= 1e-4, worst
+
+
+def test_reference_consistency_camera_gpu():
+    """unittests/testConsistencyCamera.cpp:43-190 on the GPU: 8x8 rays from a look-at viewport, step 0.25,
+    forward-mode ray gradients (d_rayDir 0-2, d_rayStart 3-5) vs the adjoint's."""
+    rng = np.random.default_rng(7500)
+    W = H = 8
+    vp = pr.Camera.viewport_from_lookat(torch.tensor([[-1.0, -0.5, 0.0]], device=dev()),
+                                        torch.tensor([[0.0, 0.0, 0.0]], device=dev()),
+                                        torch.tensor([[0.0, 0.0, 1.0]], device=dev()))
+    start, direction = pr.Camera.generate_rays(vp, math.radians(45.0), W, H)
+    worst = 0.0
+    for mode, tf in consistency_tfs(rng):
+        s = dict(W=W, H=H, filter=1, camera=0, blend=0, early_out=1.0 - 1e-5, fov_y=0.0, tf_mode=mode, tf=tf,
+                 volume=marschner_lobb(32), box_min=np.full((1, 3), -0.5), box_size=np.ones((1, 3)),
+                 step=np.full((1, 1, 1), 0.25), ray_start=start.double().cpu().numpy(),
+                 ray_dir=direction.double().cpu().numpy())
+        fd = pr.ForwardDifferencesSettings()
+        fd.D = 6
+        fd.d_rayDir = pr.int3(0, 1, 2)
+        fd.d_rayStart = pr.int3(3, 4, 5)
+        inp, color, term, g = render_fg(s, fd)
+        adj = t32(rng.standard_normal(color.shape))
+        a = pr.AdjointOutputs()
+        a.has_camera_derivatives = True
+        a.adj_camera_ray_start = torch.zeros(1, H, W, 3, device=dev())
+        a.adj_camera_ray_dir = torch.zeros(1, H, W, 3, device=dev())
+        pr.Renderer.forward_variables_to_gradients(g, adj, fd, a)
+        for label, kw in (("inversion", dict(method="inversion", path="plain")), ("checkpoint", {})):
+            b = pr.AdjointOutputs()
+            b.has_camera_derivatives = True
+            b.adj_camera_ray_start = torch.zeros(1, H, W, 3, device=dev())
+            b.adj_camera_ray_dir = torch.zeros(1, H, W, 3, device=dev())
+            pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), adj, b, **kw)
+            torch.cuda.synchronize()
+            e = max(sym_err(a.adj_camera_ray_start.cpu().numpy(), b.adj_camera_ray_start.cpu().numpy()),
+                    sym_err(a.adj_camera_ray_dir.cpu().numpy(), b.adj_camera_ray_dir.cpu().numpy()))
+            print(f"TF mode {mode}: forward vs {label} adjoint camera gradients, symmetric error {e:.1e}")
+            worst = max(worst, e)
+    assert worst <= 1e-4, worst
