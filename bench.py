@@ -17,9 +17,11 @@ Workloads (synthetic, seeded, identical on every host; diffdvr_amd/scenes.py):
      from perturbed cameras, gradient w.r.t. latitude / longitude / radius
   c5: 16 volumes {64..256}^3 with 4-32 cameras each at 128..512, linear TF per
      volume, L2, density + TF gradient, HIP Adam, 100 steps captured as one HIP graph
+  fwd: forward mode on the C2 scene, D = 7 (per-pixel step size, ray start, ray
+     direction), contracted with the L1 loss gradient
 
 Prints ONE JSON line.  Options:
-  --workload c2|c3|c4|c5  --steps K --warmup W  --gpus N (only 1)
+  --workload c2|c3|c4|c5|fwd  --steps K --warmup W  --gpus N (only 1)
   --no-cpu-baseline  --profile (short run for rocprofv3: no plain leg, no baseline)
   --path persistent|fused (c2: persistent plan, or the single-batch fused kernel)
   --checkpoint-interval K  --wave-width W  --grid G  (all 0 = autotuned; c4 uses defaults)
@@ -42,14 +44,14 @@ HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
 # gathers in the forward march, 32 B again for the adjoint, plus 8 atomic
 # read-modify-writes of 8 B for a volume gradient (TF and camera gradients are
 # privatised in LDS / registers: 0 B per sample)
-BYTES_PER_SAMPLE = {"c2": 64, "c3": 128, "c4": 64, "c5": 128}
+BYTES_PER_SAMPLE = {"c2": 64, "c3": 128, "c4": 64, "c5": 128, "fwd": 32}
 METRIC = "fwd+adjoint ray-samples/s on 1 MI355X, batched vs plain, at 1e-4 rel grad err"
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    p.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "fwd"])
     p.add_argument("--steps", type=int, default=0, help="timed steps (0 = per-workload default)")
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -89,18 +91,19 @@ def timed(fn, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
-def result_line(args, wl_name, value, ms, n_samples, config, kernel_ms, extra=None):
+def result_line(args, wl_name, value, ms, n_samples, config, kernel_ms, extra=None, kernel=None, metric=METRIC):
     achieved = n_samples * BYTES_PER_SAMPLE[wl_name] / (kernel_ms * 1e-3)
+    if kernel is None:
+        kernel = ("dvr::k_batched" if (wl_name != "c2" or args.path == "persistent")
+                  else "dvr::k_adjoint_tiled (fused)")
     r = {
-        "metric": METRIC, "value": value, "unit": "ray-samples/s", "n_gpus": 1, "steps": args.steps,
+        "metric": metric, "value": value, "unit": "ray-samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
                      "traffic": traffic_from_profiles(f"pmc_traffic_batched_{wl_name.upper()}.txt"),
-                     "kernel": "dvr::k_batched" if (wl_name != "c2" or args.path == "persistent")
-                     else "dvr::k_adjoint_tiled (fused)",
-                     "kernel_ms": kernel_ms, "bytes_per_sample": BYTES_PER_SAMPLE[wl_name]},
+                     "kernel": kernel, "kernel_ms": kernel_ms, "bytes_per_sample": BYTES_PER_SAMPLE[wl_name]},
     }
     if extra:
         r.update(extra)
@@ -565,13 +568,117 @@ def bench_c5(args):
     return result_line(args, "c5", value, ms, n_samples, config, k_ms, {"autotuned": tuned is not None})
 
 
+# ---------------------------------------------------------------------------
+# forward mode on the C2 scene (pytests/tests/stepsize/train_stepsize.py:150-200,
+# pytests/tests/camera/test_camera_optimization.py:90-125)
+# ---------------------------------------------------------------------------
+def cpu_baseline_fwd(wl, D):
+    """The FP32 dual-number restatement of the reference's forward mode on view 0."""
+    import numpy as np
+    from oracle import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    s = dict(W=wl.width, H=wl.height, filter=1, camera=2, tf_mode=1, blend=0, early_out=1 - 1e-5, fov_y=wl.fov_y,
+             volume=wl.volume.double().cpu().numpy(), box_min=np.full((1, 3), -0.5), box_size=np.ones((1, 3)),
+             step=np.full((1, 1, 1), wl.step), tf=wl.tf.double().cpu().numpy(),
+             cam_matrix=wl.viewports[:1].double().cpu().numpy())
+    t0 = time.perf_counter()
+    _, t, _ = oracle.forward_gradients(s, D, d_step=0, d_start=(1, 2, 3), d_dir=(4, 5, 6), dtype=np.float32)
+    dt = time.perf_counter() - t0
+    samples = float(t.sum())
+    return {"value": samples / dt, "unit": "ray-samples/s", "cores": threads, "kind": "port",
+            "sample": f"view 0 of the workload ({wl.width}x{wl.height}, {samples:.3g} samples, {dt:.2f} s): FP32 "
+                      f"dual-number forward mode with D={D}, the restatement of renderer_kernels.cuh:135-308"}
+
+
+def bench_fwd(args):
+    import numpy as np
+    import torch
+    from diffdvr_amd import pyrenderer as pr
+    from diffdvr_amd import scenes
+
+    dev = torch.device("cuda:0")
+    wl = scenes.workload_c2(device="cpu").to(dev)
+    B, H, W = wl.B, wl.height, wl.width
+    stream = torch.cuda.current_stream()
+    steps = args.steps or 20
+    args.steps = steps
+    D = 7
+    target = torch.empty(B, H, W, 4, device=dev)
+    tterm = torch.empty(B, H, W, dtype=torch.int32, device=dev)
+    pr.Renderer.render_forward(make_inputs(pr, wl, wl.target_tf), pr.RendererOutputs(target, tterm))
+    inp = make_inputs(pr, wl, wl.tf)
+    fd = pr.ForwardDifferencesSettings()
+    fd.D = D
+    fd.d_stepsize = 0
+    fd.d_rayStart = pr.int3(1, 2, 3)
+    fd.d_rayDir = pr.int3(4, 5, 6)
+    color = torch.empty(B, H, W, 4, device=dev)
+    term = torch.empty(B, H, W, dtype=torch.int32, device=dev)
+    grads = torch.empty(B, H, W, D, 4, device=dev)
+    outs = pr.RendererOutputs(color, term)
+    adj = pr.AdjointOutputs()
+    adj.has_stepsize_derivatives = True
+    adj.adj_stepsize = torch.zeros(B, H, W, device=dev)
+    adj.has_camera_derivatives = True
+    adj.adj_camera_ray_start = torch.zeros(B, H, W, 3, device=dev)
+    adj.adj_camera_ray_dir = torch.zeros(B, H, W, 3, device=dev)
+    n_all = float(B * H * W * 4)
+
+    def render():
+        pr.Renderer.render_forward_gradients(inp, fd, outs, grads)
+
+    def fwd_step():
+        render()
+        adj.adj_stepsize.zero_()
+        adj.adj_camera_ray_start.zero_()
+        adj.adj_camera_ray_dir.zero_()
+        g = torch.sign(color - target) / n_all
+        pr.Renderer.forward_variables_to_gradients(grads, g, fd, adj)
+
+    if args.pmc_probe:
+        for _ in range(3):
+            render()
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)   # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": src.numel() * 4,
+                          "samples_per_step": float(term.sum().item())}))
+        return None
+    for _ in range(args.warmup):
+        fwd_step()
+    torch.cuda.synchronize()
+    n_samples = float(term.sum().item())
+    ms = timed(fwd_step, steps, stream)
+    value = n_samples / (ms * 1e-3)
+    kernel_ms = float(np.mean([timed(render, 1, stream) for _ in range(max(3, steps // 2))]))
+    fcol = torch.empty_like(color)
+    fterm = torch.empty_like(term)
+    ms_forward = timed(lambda: pr.Renderer.render_forward(inp, pr.RendererOutputs(fcol, fterm), path="tiled"),
+                       steps, stream)
+    config = {"workload": "forward mode on the C2 scene: 128^3 blob volume, 64x4 texture TF, 8 views 256x256, "
+                          "step 0.25 voxel; derivatives of every pixel w.r.t. its step size, ray start and ray "
+                          "direction (D = 7), contracted with the L1 loss gradient",
+              "views": B, "image": [H, W], "samples_per_step": n_samples, "D": D}
+    extra = {"forward_only_ms": ms_forward, "derivative_cost_vs_forward": kernel_ms / ms_forward}
+    result = result_line(args, "fwd", value, ms, n_samples, config, kernel_ms, extra,
+                         kernel="dvr::k_forward_gradients",
+                         metric="forward-mode ray-samples/s on 1 MI355X (D = 7 derivatives per pixel)")
+    result["roofline"]["traffic"] = traffic_from_profiles("pmc_traffic_fwdgrad_C2.txt")
+    if not args.profile and not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline_fwd(wl, D)
+    return result
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported; this build measures one MI355X "
               f"(multi-GPU sharding is a design note, not a build target)", file=sys.stderr)
         sys.exit(2)
-    fn = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload]
+    fn = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "fwd": bench_fwd}[args.workload]
     result = fn(args)
     if result is not None:
         print(json.dumps(result))
