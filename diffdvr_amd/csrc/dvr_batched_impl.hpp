@@ -56,7 +56,7 @@ __device__ __forceinline__ int batched_pop(const BatchParams& B, int xcd, int& q
 }
 
 template <int FILTER, int TFM, int BLEND, int K, int GM>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 ? 5 : (GM == 1 ? 4 : 2)))) void k_batched(BatchParams B)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 ? 5 : 2))) void k_batched(BatchParams B)
 {
     constexpr bool ADJ = GM != 0;
     extern __shared__ float lds[];

@@ -576,17 +576,12 @@ __device__ __forceinline__ void blend_t(Acc& acc, const f4& s, float step)
 // Blend adjoint given the transmittance before the sample (t_in = 1 - a_in);
 // same derivative as blend_adjoint.  The alpha-mode clamp at 1-1e-5 follows
 // renderer_blending.cuh:136,158-159.
+// blend_adjoint_ak: the same with the Beer-Lambert (alpha, keep) of the sample
+// supplied by the caller (as alpha_keep<0> computed them in the recompute).
 template <int BLEND>
-__device__ __forceinline__ void blend_adjoint_t(float t_in, const f4& s, float step, f4& adj_acc, f4& adj_s,
-                                                float& adj_step)
+__device__ __forceinline__ void blend_adjoint_ak(float t_in, const f4& s, float step, float alpha, float keep,
+                                                 f4& adj_acc, f4& adj_s, float& adj_step)
 {
-    float alpha, keep;
-    if constexpr (BLEND == 0) {
-        alpha_keep<0>(s.w, step, alpha, keep);
-    } else {
-        alpha = tmin_(1.0f - 1e-5f, s.w * step);
-        keep = 1.0f - alpha;
-    }
     const float dotc = adj_acc.x * s.x + adj_acc.y * s.y + adj_acc.z * s.z;
     const float adj_alpha = t_in * (adj_acc.w + dotc);
     const float k = alpha * t_in;
@@ -602,6 +597,20 @@ __device__ __forceinline__ void blend_adjoint_t(float t_in, const f4& s, float s
         adj_s.w = lin ? adj_alpha * step : 0.0f;
     }
     adj_acc.w = adj_acc.w * keep - alpha * dotc;
+}
+
+template <int BLEND>
+__device__ __forceinline__ void blend_adjoint_t(float t_in, const f4& s, float step, f4& adj_acc, f4& adj_s,
+                                                float& adj_step)
+{
+    float alpha, keep;
+    if constexpr (BLEND == 0) {
+        alpha_keep<0>(s.w, step, alpha, keep);
+    } else {
+        alpha = tmin_(1.0f - 1e-5f, s.w * step);
+        keep = 1.0f - alpha;
+    }
+    blend_adjoint_ak<BLEND>(t_in, s, step, alpha, keep, adj_acc, adj_s, adj_step);
 }
 
 // Blend adjoint for one sample given the accumulated alpha before it (a_in).

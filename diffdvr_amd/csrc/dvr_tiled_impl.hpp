@@ -268,7 +268,11 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
     wantTf = CT_TF && wantTf;
     wantVol = CT_VOL && wantVol;
     wantPos = CT_POS && wantPos;
+    // Beer-Lambert segments of up to 16 samples also keep each sample's (alpha,
+    // keep) from the recompute, so the backward walk evaluates no exponential
+    constexpr bool kKeepAK = BLEND == 0 && K <= 16;
     float dv[K], tin[K];
+    float al[kKeepAK ? K : 1], kp[kKeepAK ? K : 1];
     // reverse-walk cell state for the volume / position adjoint
     CornerCache cc;
     cc.cx = kInvalidCell; cc.cy = 0; cc.cz = 0;
@@ -314,6 +318,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                     tin[j] = Tr;
                     float alpha, keep;
                     alpha_keep<BLEND>(tf_absorption<TFM>(T, dv[j]), r.step, alpha, keep);
+                    if constexpr (kKeepAK) { al[j] = alpha; kp[j] = keep; }
                     Tr *= keep;
                     if (more) pipe.advance();
                     p = pn;
@@ -329,7 +334,8 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 const f4 s = tf_eval<TFM>(T, dv[j]);
                 f4 adj_s;
                 float adj_st;
-                blend_adjoint_t<BLEND>(tin[j], s, r.step, adj_acc, adj_s, adj_st);
+                if constexpr (kKeepAK) blend_adjoint_ak<BLEND>(tin[j], s, r.step, al[j], kp[j], adj_acc, adj_s, adj_st);
+                else blend_adjoint_t<BLEND>(tin[j], s, r.step, adj_acc, adj_s, adj_st);
                 float adj_d;
                 if (wantTf) adj_d = tf_adjoint<TFM, CT_TF>(T, dv[j], adj_s, sink);
                 else {
