@@ -110,7 +110,7 @@ struct dvr_batch_plan {
         P.earlyOut = earlyOut; P.lossKind = lossKind;
         P.wantTf = (gm & 1) ? 1 : 0; P.wantVol = (gm & 2) ? 1 : 0; P.wantCam = (gm & 4) ? 1 : 0;
         P.waveShift = wave_shift_of(waveWidth);
-        P.tileW = 2 * waveWidth; P.tileH = 2 * (64 / waveWidth);
+        P.tileW = waveWidth; P.tileH = 64 / waveWidth;
         P.ldsTf = maxRC; P.usePriv = usePriv ? 1 : 0;
         return P;
     }
@@ -118,7 +118,7 @@ struct dvr_batch_plan {
     // tiles, queues, grid and checkpoint scratch for the current options
     void layout()
     {
-        const int tileW = 2 * waveWidth, tileH = 2 * (64 / waveWidth);
+        const int tileW = waveWidth, tileH = 64 / waveWidth;  // one wave tile per workgroup
         long long t = 0;
         for (auto& v : views) {
             v.tilesX = (v.W + tileW - 1) / tileW;
@@ -129,7 +129,7 @@ struct dvr_batch_plan {
         if (t >= (1LL << 31)) fail(DVR_ERR_INVALID_ARGUMENT, "batch has too many tiles");
         nTiles = (int)t;
         for (int q = 0; q <= dvr::kNumXcd; ++q) qBegin[q] = (int)((long long)nTiles * q / dvr::kNumXcd);
-        const size_t ldsG = gm ? (usePriv ? size_t(maxRC) * dvr::kBlock : size_t(maxRC)) : 0;
+        const size_t ldsG = gm ? (usePriv ? size_t(maxRC) * dvr::kBatchBlock : size_t(maxRC)) : 0;
         lds = sizeof(float) * (size_t(maxRC) + ldsG);
         int occ = 0;
         check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, 0, lds, nullptr, &occ),
@@ -138,13 +138,14 @@ struct dvr_batch_plan {
         check_hip(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         cus = prop.multiProcessorCount;
         resident = std::max(1, occ) * cus;
-        // default: two workgroups per CU (measured best for gather-bound tiles on
-        // MI355X: fewer concurrent rays per CU keep their cells in L1), within residency
-        int g = gridReq > 0 ? gridReq : std::min(resident, 2 * cus);
+        // default: two waves per SIMD (eight one-wave workgroups per CU; measured
+        // best for gather-bound tiles on MI355X: fewer concurrent rays per CU keep
+        // their cells in L1), within residency
+        int g = gridReq > 0 ? gridReq : std::min(resident, 8 * cus);
         g = std::max(1, std::min(g, std::max(nTiles, 1)));
         grid = g;
         cap = std::max(1, std::min((int)std::ceil(maxSamples / K), 1 << 14));
-        const size_t need = gm ? size_t(grid) * cap * dvr::kBlock : 0;
+        const size_t need = gm ? size_t(grid) * cap * dvr::kBatchBlock : 0;
         if (need > ckFloats) {
             dev_free(dCk);
             dCk = nullptr;
@@ -480,8 +481,8 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
                 std::vector<int> grids;
                 if (gridReq > 0) grids.push_back(gridReq);
                 else
-                    for (int k = 1; k <= 4; ++k)
-                        if (k * cus <= resident) grids.push_back(k * cus);
+                    for (int k = 1; k <= 4; ++k)  // 1..4 waves per SIMD
+                        if (4 * k * cus <= resident) grids.push_back(4 * k * cus);
                 if (grids.empty()) grids.push_back(resident);
                 for (int g : grids) {
                     const float ms = time_it(cur.K, cur.ww, g);

@@ -27,11 +27,11 @@ template <int TFM>
 __device__ __forceinline__ void batched_flush_tf(const BatchParams& B, int tfi, const float* s_g, int RC, bool priv)
 {
     float* g = B.tfs[tfi].adj;
-    for (int e = threadIdx.x; e < RC; e += kBlock) {
+    for (int e = threadIdx.x; e < RC; e += kBatchBlock) {
         float v;
         if (priv) {
             v = 0.0f;
-            for (int t = 0; t < kBlock; ++t) v += s_g[e * kBlock + t];
+            for (int t = 0; t < kBatchBlock; ++t) v += s_g[e * kBatchBlock + t];
         } else {
             v = s_g[e];
         }
@@ -56,7 +56,7 @@ __device__ __forceinline__ int batched_pop(const BatchParams& B, int xcd, int& q
 }
 
 template <int FILTER, int TFM, int BLEND, int K, int GM>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 ? 5 : 2))) void k_batched(BatchParams B)
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 ? 5 : 2))) void k_batched(BatchParams B)
 {
     constexpr bool ADJ = GM != 0;
     extern __shared__ float lds[];
@@ -64,11 +64,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
     float* s_tf = lds;
     float* s_g = lds + B.ldsTf;
     const int xcd = int(blockIdx.x) % kNumXcd;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     const int ww = 1 << B.waveShift;
-    const int lx = (wave & 1) * ww + (lane & (ww - 1));
-    const int ly = (wave >> 1) * (64 >> B.waveShift) + (lane >> B.waveShift);
-    float* ck = ADJ ? B.ck + size_t(blockIdx.x) * size_t(B.ckCap) * kBlock + threadIdx.x : nullptr;
+    const int lx = lane & (ww - 1);
+    const int ly = lane >> B.waveShift;
+    float* ck = ADJ ? B.ck + size_t(blockIdx.x) * size_t(B.ckCap) * kBatchBlock + threadIdx.x : nullptr;
     const bool priv = B.usePriv != 0;
     int qoff = 0;
     int curTf = -1, curRC = 0;
@@ -99,10 +99,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
             curTf = tfi;
             curRC = tf.R * tf.C;
             curWantTf = ADJ && B.wantTf && tf.adj != nullptr;
-            for (int e = threadIdx.x; e < curRC; e += kBlock) s_tf[e] = gld(tf.p, unsigned(e));
+            for (int e = threadIdx.x; e < curRC; e += kBatchBlock) s_tf[e] = gld(tf.p, unsigned(e));
             if (curWantTf) {
-                const int ng = priv ? curRC * kBlock : curRC;
-                for (int e = threadIdx.x; e < ng; e += kBlock) s_g[e] = 0.0f;
+                const int ng = priv ? curRC * kBatchBlock : curRC;
+                for (int e = threadIdx.x; e < ng; e += kBatchBlock) s_g[e] = 0.0f;
             }
             __syncthreads();
         }
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
         constexpr int kC = tf_cols<TFM>();  // validated: every TF of the plan has this many columns
         const Tf T{s_tf, curRC / kC, kC, kC, 1};
         SinkTiled<TFM> sink;
-        sink.init(s_g, kC, priv);
+        sink.init(s_g, kC, priv, kBatchBlock);
         const bool wantCam = ADJ && (GM & 4) && B.wantCam && V.acam != nullptr;
         float cv[9];
 #pragma unroll
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
             const Vol VV{vol.p, vol.X, vol.Y, vol.Z, vol.Y * vol.Z, vol.Z, 1};
             Acc acc;
             int n;
-            march_forward<FILTER, TFM, BLEND>(VV, T, r, B.earlyOut, -1, acc, n, ck, kBlock, K * B.ckEvery,
+            march_forward<FILTER, TFM, BLEND>(VV, T, r, B.earlyOut, -1, acc, n, ck, kBatchBlock, K * B.ckEvery,
                                               ADJ ? B.ckCap : 0);
             const unsigned pix = unsigned(y * V.W + x);
             gst(V.color, 4 * pix, acc.r);
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 0 
                 f3 adj_start = mk3(0.f, 0.f, 0.f), adj_dir = mk3(0.f, 0.f, 0.f);
                 float adj_step = 0.0f;
                 const bool wantVol = B.wantVol && vol.adj != nullptr;
-                march_reverse<FILTER, TFM, BLEND, K, GM>(VV, T, r, n, adj_acc, ck, kBlock, B.ckCap, B.ckEvery, sink,
+                march_reverse<FILTER, TFM, BLEND, K, GM>(VV, T, r, n, adj_acc, ck, kBatchBlock, B.ckCap, B.ckEvery, sink,
                                                          curWantTf,
                                                          wantVol, wantCam, vol.adj, vol.Y * vol.Z, vol.Z, 1,
                                                          adj_start, adj_dir, adj_step);
@@ -225,9 +225,9 @@ template <int FILTER, int TFM, int BLEND, int K, int GM>
 static hipError_t batched_k(const BatchParams& P, int grid, size_t lds, hipStream_t s, int* occ)
 {
     auto kern = k_batched<FILTER, TFM, BLEND, K, GM>;
-    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, kern, kBlock, lds);
+    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, kern, kBatchBlock, lds);
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, P);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBatchBlock), lds, s, P);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@ namespace dvr {
 constexpr int kTileW = 16;        // pixels per workgroup tile (x)
 constexpr int kTileH = 16;        // pixels per workgroup tile (y)
 constexpr int kBlock = 256;       // threads per workgroup = 4 waves of 8x8 rays
+constexpr int kBatchBlock = 64;   // batched persistent kernel: one wave per workgroup, one wave tile per pop
 constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kMaxTfLds = 4096;   // max R*C floats of a TF staged in LDS by the tiled kernels
 constexpr int kMaxSamples = 1 << 24;  // hard cap on samples per ray (termination guarantee)
@@ -116,7 +117,7 @@ struct BatchParams {
     const BVol* vols;
     const BTf* tfs;
     unsigned* queues;           // kNumXcd tile counters, zeroed before every launch
-    float* ck;                  // checkpoints [workgroup][cap][kBlock]
+    float* ck;                  // checkpoints [workgroup][cap][kBatchBlock]
     float* lossOut;             // scalar loss accumulator or NULL
     int nViews, nTiles, ckCap;
     int ckEvery;                // checkpoint interval = K * ckEvery samples
@@ -125,7 +126,7 @@ struct BatchParams {
     int lossKind;               // DVR_LOSS_*
     int wantTf, wantVol, wantCam;
     int waveShift;              // log2 of the wave tile width; wave tile = (1<<s) x (64>>s) rays
-    int tileW, tileH;           // workgroup tile = 2 x 2 wave tiles
+    int tileW, tileH;           // workgroup tile = one wave tile
     int ldsTf;                  // floats of LDS per TF table (max R*C over the TFs)
     int usePriv;                // Gaussian TF gradient in per-thread LDS slices
 };

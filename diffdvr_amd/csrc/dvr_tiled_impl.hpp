@@ -109,14 +109,15 @@ template <int TFM>
 struct SinkTiled {
     float* tab;   // workgroup table (R*C) or private slice base (priv)
     int C;
+    int nthreads; // threads of the workgroup (slice stride of the private slices)
     bool priv;
     int r0, r1;
     float a0[5], a1[5];
     float id0, id1;
 
-    __device__ __forceinline__ void init(float* table, int C_, bool usePriv)
+    __device__ __forceinline__ void init(float* table, int C_, bool usePriv, int threads = kBlock)
     {
-        tab = table; C = C_; priv = usePriv;
+        tab = table; C = C_; priv = usePriv; nthreads = threads;
         r0 = r1 = -1;
 #pragma unroll
         for (int k = 0; k < 5; ++k) { a0[k] = 0.f; a1[k] = 0.f; }
@@ -127,7 +128,7 @@ struct SinkTiled {
         if constexpr (TFM == 0) {
             if (c == 0) id0 += v; else id1 += v;
         } else {
-            if (priv) tab[(r * C + c) * kBlock + threadIdx.x] += v;
+            if (priv) tab[(r * C + c) * nthreads + threadIdx.x] += v;
             else atomicAdd(&tab[r * C + c], v);
         }
     }
