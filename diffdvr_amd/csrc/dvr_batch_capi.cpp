@@ -97,6 +97,15 @@ struct dvr_batch_plan {
         dev_free(dViews); dev_free(dVols); dev_free(dTfs); dev_free(dQueues); dev_free(dSegsAll); dev_free(dCk);
     }
 
+    // floats between the TF-gradient table copies (odd: the copies start in different LDS banks);
+    // 0 (one table) when the copies would take more than 16 KiB of LDS per workgroup, which keeps
+    // eight one-wave workgroups resident per CU
+    int copy_stride() const
+    {
+        const int st = maxRC | 1;
+        return size_t(dvr::kTfCopies) * st * sizeof(float) <= 16384 ? st : 0;
+    }
+
     // register segment length of checkpoint interval K (64 = two 32-sample segments)
     int segK() const { return K > 32 ? 32 : K; }
 
@@ -112,6 +121,7 @@ struct dvr_batch_plan {
         P.waveShift = wave_shift_of(waveWidth);
         P.tileW = waveWidth; P.tileH = 64 / waveWidth;
         P.ldsTf = maxRC; P.usePriv = usePriv ? 1 : 0;
+        P.tfCopyStride = (gm && !usePriv) ? copy_stride() : 0;
         return P;
     }
 
@@ -129,7 +139,10 @@ struct dvr_batch_plan {
         if (t >= (1LL << 31)) fail(DVR_ERR_INVALID_ARGUMENT, "batch has too many tiles");
         nTiles = (int)t;
         for (int q = 0; q <= dvr::kNumXcd; ++q) qBegin[q] = (int)((long long)nTiles * q / dvr::kNumXcd);
-        const size_t ldsG = gm ? (usePriv ? size_t(maxRC) * dvr::kBatchBlock : size_t(maxRC)) : 0;
+        const size_t ldsG =
+            gm ? (usePriv ? size_t(maxRC) * dvr::kBatchBlock
+                          : (copy_stride() > 0 ? size_t(dvr::kTfCopies) * copy_stride() : size_t(maxRC)))
+               : 0;
         lds = sizeof(float) * (size_t(maxRC) + ldsG);
         int occ = 0;
         check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, 0, lds, nullptr, &occ),

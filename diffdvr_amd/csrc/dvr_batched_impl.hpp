@@ -32,6 +32,10 @@ __device__ __forceinline__ void batched_flush_tf(const BatchParams& B, int tfi, 
         if (priv) {
             v = 0.0f;
             for (int t = 0; t < kBatchBlock; ++t) v += s_g[e * kBatchBlock + t];
+        } else if (B.tfCopyStride > 0) {
+            v = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kTfCopies; ++k) v += s_g[k * B.tfCopyStride + e];
         } else {
             v = s_g[e];
         }
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
             curWantTf = ADJ && B.wantTf && tf.adj != nullptr;
             for (int e = threadIdx.x; e < curRC; e += kBatchBlock) s_tf[e] = gld(tf.p, unsigned(e));
             if (curWantTf) {
-                const int ng = priv ? curRC * kBatchBlock : curRC;
+                const int ng = priv ? curRC * kBatchBlock : (B.tfCopyStride > 0 ? kTfCopies * B.tfCopyStride : curRC);
                 for (int e = threadIdx.x; e < ng; e += kBatchBlock) s_g[e] = 0.0f;
             }
             __syncthreads();
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
         constexpr int kC = tf_cols<TFM>();  // validated: every TF of the plan has this many columns
         const Tf T{s_tf, curRC / kC, kC, kC, 1};
         SinkTiled<TFM> sink;
-        sink.init(s_g, kC, priv, kBatchBlock);
+        sink.init(s_g, kC, priv, kBatchBlock, priv ? 0 : B.tfCopyStride);
         const bool wantCam = ADJ && (GM & 4) && B.wantCam && V.acam != nullptr;
         float cv[9];
 #pragma unroll

@@ -11,6 +11,7 @@ constexpr int kTileW = 16;        // pixels per workgroup tile (x)
 constexpr int kTileH = 16;        // pixels per workgroup tile (y)
 constexpr int kBlock = 256;       // threads per workgroup = 4 waves of 8x8 rays
 constexpr int kBatchBlock = 64;   // batched persistent kernel: one wave per workgroup, one wave tile per pop
+constexpr int kTfCopies = 8;      // batched kernel: copies of the LDS TF-gradient table, lane & 7 selects one
 constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kMaxTfLds = 4096;   // max R*C floats of a TF staged in LDS by the tiled kernels
 constexpr int kMaxSamples = 1 << 24;  // hard cap on samples per ray (termination guarantee)
@@ -129,6 +130,7 @@ struct BatchParams {
     int tileW, tileH;           // workgroup tile = one wave tile
     int ldsTf;                  // floats of LDS per TF table (max R*C over the TFs)
     int usePriv;                // Gaussian TF gradient in per-thread LDS slices
+    int tfCopyStride;           // floats between the kTfCopies TF-gradient tables (0: one table)
 };
 
 // One segment of a zero-fill list (gradients, counters) cleared by a kernel

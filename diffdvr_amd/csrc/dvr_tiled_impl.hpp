@@ -101,13 +101,16 @@ __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const R
 //   identity : two registers per lane, wave-reduced at the end
 //   texture / linear : per-lane cache of the two most recent rows (densities
 //       change slowly along a ray), flushed into a workgroup LDS table with
-//       LDS atomics only when the row changes
+//       LDS atomics only when a row leaves (a move by one texel keeps the
+//       shared row); the batched kernel spreads the atomics of a wave over
+//       kTfCopies copies of the table (neighbouring rays hit the same rows)
 //   gaussian : every sample touches every entry; per-thread private LDS slices
 //       (entry-major, conflict-free) when they fit, else LDS atomics
 // ---------------------------------------------------------------------------
 template <int TFM>
 struct SinkTiled {
     float* tab;   // workgroup table (R*C) or private slice base (priv)
+    float* tabc;  // this lane's copy of the table (copies: lanes of one wave spread their LDS atomics)
     int C;
     int nthreads; // threads of the workgroup (slice stride of the private slices)
     bool priv;
@@ -115,9 +118,11 @@ struct SinkTiled {
     float a0[5], a1[5];
     float id0, id1;
 
-    __device__ __forceinline__ void init(float* table, int C_, bool usePriv, int threads = kBlock)
+    // copyStride > 0: kTfCopies tables copyStride floats apart, lane & (kTfCopies-1) adds to its own
+    __device__ __forceinline__ void init(float* table, int C_, bool usePriv, int threads = kBlock, int copyStride = 0)
     {
         tab = table; C = C_; priv = usePriv; nthreads = threads;
+        tabc = table + (copyStride > 0 ? int(threadIdx.x & (kTfCopies - 1)) * copyStride : 0);
         r0 = r1 = -1;
 #pragma unroll
         for (int k = 0; k < 5; ++k) { a0[k] = 0.f; a1[k] = 0.f; }
@@ -129,7 +134,7 @@ struct SinkTiled {
             if (c == 0) id0 += v; else id1 += v;
         } else {
             if (priv) tab[(r * C + c) * nthreads + threadIdx.x] += v;
-            else atomicAdd(&tab[r * C + c], v);
+            else atomicAdd(&tabc[r * C + c], v);
         }
     }
     template <int NC>
@@ -138,7 +143,7 @@ struct SinkTiled {
         if (r >= 0) {
 #pragma unroll
             for (int c = 0; c < NC; ++c)
-                if (a[c] != 0.0f) atomicAdd(&tab[r * C + c], a[c]);
+                if (a[c] != 0.0f) atomicAdd(&tabc[r * C + c], a[c]);
         }
 #pragma unroll
         for (int k = 0; k < 5; ++k) a[k] = 0.f;
@@ -147,6 +152,20 @@ struct SinkTiled {
     __device__ __forceinline__ void add_pair(int i0, float w0, int i1, float w1, const f4& adj, float p0, float p1)
     {
         constexpr int NC = POS ? 5 : 4;
+        if (i0 != r0) {
+            // a move by one texel keeps the accumulator of the row both pairs share
+            if (i0 == r1 && r1 >= 0) {
+                flush_row<NC>(r0, a0);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) { a0[k] = a1[k]; a1[k] = 0.f; }
+                r0 = i0; r1 = -1;
+            } else if (i1 == r0 && i1 != i0) {
+                flush_row<NC>(r1, a1);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) { a1[k] = a0[k]; a0[k] = 0.f; }
+                r1 = i1; r0 = -1;
+            }
+        }
         if (i0 != r0) { flush_row<NC>(r0, a0); r0 = i0; }
         a0[0] += adj.x * w0; a0[1] += adj.y * w0; a0[2] += adj.z * w0; a0[3] += adj.w * w0;
         if constexpr (POS) a0[4] += p0;
