@@ -142,8 +142,7 @@ struct SinkTiled {
     {
         if (r >= 0) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
-                if (a[c] != 0.0f) atomicAdd(&tabc[r * C + c], a[c]);
+            for (int c = 0; c < NC; ++c) atomicAdd(&tabc[r * C + c], a[c]);
         }
 #pragma unroll
         for (int k = 0; k < 5; ++k) a[k] = 0.f;
