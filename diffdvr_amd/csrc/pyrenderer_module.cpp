@@ -602,6 +602,77 @@ PYBIND11_MODULE(_pyrenderer, m)
           },
           py::arg("color_input"), py::arg("gradients_input"), py::arg("color_reference"), py::arg("difference_out"),
           py::arg("gradients_out"), py::arg("reduce"));
+    // volume files, synthetic volumes, histogram and TF builders (host buffers by address)
+    m.def("cvol_read_header", [](const std::string& path) {
+        dvr_cvol_header h;
+        check(dvr_cvol_read_header(path.c_str(), &h));
+        return py::make_tuple(py::make_tuple(h.size[0], h.size[1], h.size[2]),
+                              py::make_tuple(h.voxel_size[0], h.voxel_size[1], h.voxel_size[2]), h.type,
+                              h.compressed != 0);
+    });
+    m.def("cvol_read", [](const std::string& path, uintptr_t data, int64_t count) {
+        check(dvr_cvol_read(path.c_str(), reinterpret_cast<float*>(data), count));
+    });
+    m.def("cvol_write", [](const std::string& path, uintptr_t data, std::vector<int64_t> size, std::vector<double> world,
+                           int compression) {
+        if (size.size() != 3 || world.size() != 3) throw std::invalid_argument("size and world size need 3 entries");
+        check(dvr_cvol_write(path.c_str(), reinterpret_cast<const float*>(data), size.data(), world.data(), compression));
+    });
+    m.def("volume_implicit", [](int equation, int resolution, std::vector<std::string> names, std::vector<float> values,
+                                uintptr_t data) {
+        if (names.size() != values.size()) throw std::invalid_argument("names and values differ in length");
+        std::vector<const char*> cn;
+        for (auto& n : names) cn.push_back(n.c_str());
+        check(dvr_volume_implicit(equation, resolution, (int)cn.size(), cn.data(), values.data(),
+                                  reinterpret_cast<float*>(data)));
+    });
+    m.def("volume_histogram", [](uintptr_t data, std::vector<int64_t> size, std::vector<int64_t> strides) {
+        if (size.size() != 3 || strides.size() != 3) throw std::invalid_argument("size and strides need 3 entries");
+        std::vector<float> bins(DVR_HISTOGRAM_BINS);
+        float mn = 0, mx = 0;
+        uint32_t nnz = 0;
+        check(dvr_volume_histogram(reinterpret_cast<const float*>(data), size.data(), strides.data(), bins.data(), &mn,
+                                   &mx, &nnz));
+        return py::make_tuple(bins, mn, mx, nnz);
+    });
+    m.def("rgb_to_lab", [](float r, float g, float b) {
+        const float in[3] = {r, g, b};
+        float out[3];
+        dvr_rgb_to_lab(in, out);
+        return py::make_tuple(out[0], out[1], out[2]);
+    });
+    m.def("lab_to_rgb", [](float l, float a, float b) {
+        const float in[3] = {l, a, b};
+        float out[3];
+        dvr_lab_to_rgb(in, out);
+        return py::make_tuple(out[0], out[1], out[2]);
+    });
+    m.def("tf_assemble_from_settings", [](std::vector<float> color_lab, std::vector<float> color_pos,
+                                          std::vector<float> opacity, std::vector<float> opacity_pos, float min_density,
+                                          float max_density, float opacity_scaling, bool purge) {
+        if (color_lab.size() != 3 * color_pos.size()) throw std::invalid_argument("colour values need 3 entries each");
+        if (opacity.size() != opacity_pos.size()) throw std::invalid_argument("opacity values and positions differ");
+        int32_t count = 0;
+        check(dvr_tf_assemble_from_settings(color_lab.data(), color_pos.data(), (int32_t)color_pos.size(),
+                                            opacity.data(), opacity_pos.data(), (int32_t)opacity.size(), min_density,
+                                            max_density, opacity_scaling, purge ? 1 : 0, nullptr, 0, &count));
+        std::vector<dvr_tf_point> pts((size_t)count);
+        check(dvr_tf_assemble_from_settings(color_lab.data(), color_pos.data(), (int32_t)color_pos.size(),
+                                            opacity.data(), opacity_pos.data(), (int32_t)opacity.size(), min_density,
+                                            max_density, opacity_scaling, purge ? 1 : 0, pts.data(), count, &count));
+        py::list out;
+        for (auto& p : pts) out.append(py::make_tuple(p.pos, py::make_tuple(p.val[0], p.val[1], p.val[2], p.val[3])));
+        return out;
+    });
+    m.def("tf_texture_from_points", [](std::vector<float> pos, std::vector<float> val, int resolution, uintptr_t out) {
+        if (val.size() != 4 * pos.size()) throw std::invalid_argument("values need 4 entries per point");
+        std::vector<dvr_tf_point> pts(pos.size());
+        for (size_t i = 0; i < pos.size(); ++i) {
+            pts[i].pos = pos[i];
+            for (int c = 0; c < 4; ++c) pts[i].val[c] = val[4 * i + c];
+        }
+        check(dvr_tf_texture_from_points(pts.data(), (int32_t)pts.size(), resolution, reinterpret_cast<float*>(out)));
+    });
     m.def("checkpoint_buffer_elements",
           [](const RendererInputs& inputs, int K) {
               PackedInputs p = pack_inputs(inputs);

@@ -12,11 +12,16 @@ Extensions (keyword-only, defaults keep the reference behaviour):
   * ``Renderer.render_adjoint(..., method=, path=, checkpoint_interval=, checkpoints=)``
   * ``Renderer.render_fused(...)``: forward + L1/L2 image loss + adjoint in one launch.
 
+``Volume`` (.cvol files, implicit datasets, mipmaps, histogram) and ``TFUtils``
+(TF editor points, piecewise / texture tensors) read and build host data through
+the C-ABI.
+
 There is no CPU rendering path: every render call needs HIP device tensors
 and raises ``RuntimeError`` otherwise.
 """
 from __future__ import annotations
 
+import enum
 import math
 
 import numpy as np
@@ -239,8 +244,201 @@ class Camera:
         return Camera.viewport_from_lookat(origin + c, c, up)
 
 
+# ---------------------------------------------------------------------------
+# Volume (renderer/volume.h:42-205, bindings/bindings.cpp:234-304)
+# ---------------------------------------------------------------------------
+class ImplicitEquation(enum.IntEnum):
+    """Volume::ImplicitEquation (bindings/bindings.cpp:234-249)."""
+    MarschnerLobb = 0
+    Cube = 1
+    Sphere = 2
+    InverseSphere = 3
+    DingDong = 4
+    Endrass = 5
+    Barth = 6
+    Heart = 7
+    Kleine = 8
+    Cassini = 9
+    Steiner = 10
+    CrossCap = 11
+    Kummer = 12
+    Blobbly = 13
+    Tube = 14
+
+
+def _x_fastest(flat: torch.Tensor, X: int, Y: int, Z: int) -> torch.Tensor:
+    """1*X*Y*Z view of a dense x-fastest buffer (strides XYZ, 1, X, X*Y; renderer/volume.cpp:21-33)."""
+    return flat.as_strided((1, X, Y, Z), (X * Y * Z, 1, X, X * Y))
+
+
+class Volume:
+    """A density volume with mipmap levels, on the CPU and (after copy_to_gpu) on the device.
+
+    Levels are float32 tensors of shape 1*X*Y*Z with x the fastest axis, as the reference's
+    Volume blob (the renderer accepts any strides)."""
+
+    def __init__(self, filename: str | None = None):
+        self._world = [1.0, 1.0, 1.0]
+        self._cpu: list[torch.Tensor | None] = []
+        self._gpu: list[torch.Tensor | None] = []
+        if filename is not None:
+            (X, Y, Z), voxel, _, _ = _n.cvol_read_header(str(filename))
+            flat = torch.empty(X * Y * Z, dtype=torch.float32)
+            _n.cvol_read(str(filename), flat.data_ptr(), flat.numel())
+            self._cpu = [_x_fastest(flat, X, Y, Z)]
+            self._gpu = [None]
+            self._world = [voxel[0] * X, voxel[1] * Y, voxel[2] * Z]
+
+    @staticmethod
+    def _from_level0(t: torch.Tensor) -> "Volume":
+        v = Volume()
+        v._cpu = [t]
+        v._gpu = [None]
+        return v
+
+    @property
+    def world_size(self) -> real3:
+        return real3(*[float(w) for w in self._world])
+
+    def set_world_size(self, x: float, y: float, z: float) -> None:
+        self._world = [float(x), float(y), float(z)]
+
+    @property
+    def resolution(self) -> int3:
+        _, X, Y, Z = self._cpu[0].shape
+        return int3(int(X), int(Y), int(Z))
+
+    def create_mipmap_level(self, level: int) -> None:
+        """Level l has size max(1, N // (l+1)) per axis, averaged with adaptive average pooling
+        (renderer/volume.cpp:359-365, 421-432)."""
+        if level < 0:
+            raise ValueError(f"level has to be non-zero, but is {level}")
+        while len(self._cpu) <= level:
+            self._cpu.append(None)
+            self._gpu.append(None)
+        if self._cpu[level] is not None:
+            return
+        _, X, Y, Z = self._cpu[0].shape
+        size = (max(1, X // (level + 1)), max(1, Y // (level + 1)), max(1, Z // (level + 1)))
+        pooled = torch.nn.functional.adaptive_avg_pool3d(self._cpu[0], size)
+        flat = torch.empty(size[0] * size[1] * size[2], dtype=torch.float32)
+        lvl = _x_fastest(flat, *size)
+        lvl.copy_(pooled)
+        self._cpu[level] = lvl
+
+    @staticmethod
+    def create_implicit(name: ImplicitEquation, resolution: int, **kwargs) -> "Volume":
+        """Volume::createImplicitDataset (renderer/volume_implicit.cpp:218-236)."""
+        R = int(resolution)
+        flat = torch.empty(R * R * R, dtype=torch.float32)
+        names = list(kwargs.keys())
+        _n.volume_implicit(int(name), R, names, [float(kwargs[k]) for k in names], flat.data_ptr())
+        return Volume._from_level0(_x_fastest(flat, R, R, R))
+
+    @staticmethod
+    def from_numpy(array) -> "Volume":
+        """A volume from a 3D float32 array indexed [x][y][z] (bindings/bindings.cpp:269-289)."""
+        a = np.asarray(array)
+        if a.dtype != np.float32:
+            raise RuntimeError("Incompatible format: expected a real array!")
+        if a.ndim != 3:
+            raise RuntimeError("Incompatible buffer dimension, expected 3!")
+        X, Y, Z = a.shape
+        flat = torch.empty(X * Y * Z, dtype=torch.float32)
+        _x_fastest(flat, X, Y, Z)[0].copy_(torch.from_numpy(np.ascontiguousarray(a)))
+        return Volume._from_level0(_x_fastest(flat, X, Y, Z))
+
+    def copy_to_gpu(self) -> None:
+        for i, t in enumerate(self._cpu):
+            if t is not None:
+                g = torch.empty_strided(t.shape, t.stride(), dtype=t.dtype, device="cuda")
+                g.copy_(t)
+                self._gpu[i] = g
+
+    def save(self, filename: str, compression: int = 0) -> None:
+        """Volume::save: float voxels, uncompressed (renderer/volume.cpp:86-158)."""
+        t = self._cpu[0]
+        _, X, Y, Z = t.shape
+        flat = torch.empty(X * Y * Z, dtype=torch.float32)
+        _x_fastest(flat, X, Y, Z).copy_(t)
+        _n.cvol_write(str(filename), flat.data_ptr(), [X, Y, Z], list(self._world), int(compression))
+
+    def getDataCpu(self, level: int) -> torch.Tensor:
+        if level >= len(self._cpu) or self._cpu[level] is None:
+            raise RuntimeError(f"Mipmap Level {level} was not created")
+        return self._cpu[level]
+
+    def getDataGpu(self, level: int) -> torch.Tensor:
+        if level >= len(self._cpu) or self._cpu[level] is None:
+            raise RuntimeError(f"Mipmap Level {level} was not created")
+        if self._gpu[level] is None:
+            self.copy_to_gpu()
+        return self._gpu[level]
+
+    def extract_histogram(self):
+        """Volume::extractHistogram (renderer/volume.cpp:287-356): (bins[512], min, max, non-zero count)."""
+        t = self._cpu[0][0]
+        bins, mn, mx, nnz = _n.volume_histogram(t.data_ptr(), list(t.shape), list(t.stride()))
+        return bins, mn, mx, nnz
+
+
+# ---------------------------------------------------------------------------
+# TF builders (renderer/tf_utils.cpp:12-172, bindings/bindings.cpp:560-569)
+# ---------------------------------------------------------------------------
+class TFPoint:
+    """renderer::TFPoint: density position and (r, g, b, opacity)."""
+
+    def __init__(self, pos: float = 0.0, val: real4 | None = None):
+        self.pos = float(pos)
+        self.val = val if val is not None else real4(0.0, 0.0, 0.0, 0.0)
+
+    def __repr__(self):
+        return f"TFPoint(pos={self.pos}, val=({self.val.x}, {self.val.y}, {self.val.z}, {self.val.w}))"
+
+
+class TFUtils:
+    @staticmethod
+    def assemble_from_settings(color_values_lab, color_positions, opacity_values, opacity_positions,
+                               min_density: float, max_density: float, opacity_scaling: float,
+                               purge_zero_opacity_regions: bool) -> list:
+        """Merges the editor's colour axis (L*a*b*, real3) and opacity axis into TF points."""
+        lab = []
+        for c in color_values_lab:
+            lab += [float(c.x), float(c.y), float(c.z)] if hasattr(c, "x") else [float(v) for v in c]
+        pts = _n.tf_assemble_from_settings(lab, [float(v) for v in color_positions],
+                                           [float(v) for v in opacity_values], [float(v) for v in opacity_positions],
+                                           float(min_density), float(max_density), float(opacity_scaling),
+                                           bool(purge_zero_opacity_regions))
+        return [TFPoint(p, real4(*v)) for p, v in pts]
+
+    @staticmethod
+    def get_piecewise_tensor(points) -> torch.Tensor:
+        """1*N*5 tensor (r, g, b, opacity, position) of a linear TF (renderer/tf_utils.cpp:120-135)."""
+        rows = [[p.val.x, p.val.y, p.val.z, p.val.w, p.pos] for p in points]
+        return torch.tensor([rows], dtype=torch.float32)
+
+    @staticmethod
+    def get_texture_tensor(points, resolution: int) -> torch.Tensor:
+        """1*resolution*4 texture sampled at the texel centres (renderer/tf_utils.cpp:137-172)."""
+        out = torch.empty(1, int(resolution), 4, dtype=torch.float32)
+        vals = []
+        for p in points:
+            vals += [p.val.x, p.val.y, p.val.z, p.val.w]
+        _n.tf_texture_from_points([p.pos for p in points], vals, int(resolution), out.data_ptr())
+        return out
+
+    @staticmethod
+    def rgb_to_lab(rgb) -> real3:
+        return real3(*_n.rgb_to_lab(float(rgb.x), float(rgb.y), float(rgb.z)))
+
+    @staticmethod
+    def lab_to_rgb(lab) -> real3:
+        return real3(*_n.lab_to_rgb(float(lab.x), float(lab.y), float(lab.z)))
+
+
 __all__ = [
     "real3", "real4", "int2", "int3", "VolumeFilterMode", "CameraMode", "TFMode", "BlendMode", "Orientation",
     "CameraPerPixelRays", "CameraReferenceFrame", "mat4", "RendererInputs", "RendererOutputs", "AdjointOutputs",
     "ForwardDifferencesSettings", "GpuTimer", "sync", "use_double_precision", "Renderer", "Camera",
+    "ImplicitEquation", "Volume", "TFPoint", "TFUtils",
 ]

@@ -222,6 +222,83 @@ dvr_status dvr_forward_variables_to_gradients(const dvr_tensor* forward_variable
 dvr_status dvr_compare_to_image(const dvr_tensor* color, const dvr_tensor* gradients, const dvr_tensor* reference,
                                 dvr_tensor* difference_out, dvr_tensor* gradients_out, int32_t reduce, void* stream);
 
+/* ---- volume files, synthetic volumes, TF builders (host memory) ----------
+ * Voxel buffers are dense float32 host arrays with x fastest: index
+ * x + X*(y + Y*z), the layout of renderer::Volume's blob (renderer/volume.cpp:21-33). */
+
+#define DVR_CVOL_UCHAR 0  /* normalised by 1/255 on read */
+#define DVR_CVOL_USHORT 1 /* normalised by 1/65535 on read */
+#define DVR_CVOL_FLOAT 2
+
+/* The 64-byte .cvol header (renderer/volume.cpp:75-84). */
+typedef struct dvr_cvol_header {
+    int64_t size[3];       /* X, Y, Z */
+    double voxel_size[3];  /* world size = voxel_size * size */
+    int32_t type;          /* DVR_CVOL_* */
+    int32_t compressed;    /* LZ4 chunks follow the header */
+} dvr_cvol_header;
+
+/* Volume::Volume(filename) (renderer/volume.cpp:160-285), split into header and voxels.
+ * LZ4-compressed files return DVR_ERR_UNSUPPORTED. */
+dvr_status dvr_cvol_read_header(const char* path, dvr_cvol_header* header);
+dvr_status dvr_cvol_read(const char* path, float* data, int64_t count);
+
+/* Volume::save (renderer/volume.cpp:86-158): float voxels, voxel size = world size / size.
+ * Only compression 0 (uncompressed) is written; 1..9 return DVR_ERR_UNSUPPORTED. */
+dvr_status dvr_cvol_write(const char* path, const float* data, const int64_t size[3], const double world_size[3],
+                          int32_t compression);
+
+/* Volume::ImplicitEquation (renderer/volume.h:163-181, bindings/bindings.cpp:234-249). */
+#define DVR_IMPLICIT_MARSCHNER_LOBB 0 /* parameters "fM", "alpha" */
+#define DVR_IMPLICIT_CUBE 1           /* parameter "scale" */
+#define DVR_IMPLICIT_SPHERE 2
+#define DVR_IMPLICIT_INVERSE_SPHERE 3
+#define DVR_IMPLICIT_DING_DONG 4
+#define DVR_IMPLICIT_ENDRASS 5
+#define DVR_IMPLICIT_BARTH 6
+#define DVR_IMPLICIT_HEART 7
+#define DVR_IMPLICIT_KLEINE 8
+#define DVR_IMPLICIT_CASSINI 9        /* parameter "a" */
+#define DVR_IMPLICIT_STEINER 10
+#define DVR_IMPLICIT_CROSS_CAP 11
+#define DVR_IMPLICIT_KUMMER 12
+#define DVR_IMPLICIT_BLOBBY 13
+#define DVR_IMPLICIT_TUBE 14
+
+/* Volume::createImplicitDataset (renderer/volume_implicit.cpp:218-236, renderer/volume.cpp:369-387):
+ * resolution^3 voxels sampled on the equation's box; named float parameters. */
+dvr_status dvr_volume_implicit(int32_t equation, int32_t resolution, int32_t nparams, const char* const* names,
+                               const float* values, float* data);
+
+/* Volume::extractHistogram (renderer/volume.cpp:287-356): 512 bins over the non-zero voxels
+ * (fractions summing to 1), density minimum / maximum, non-zero count.  Any element strides. */
+#define DVR_HISTOGRAM_BINS 512
+dvr_status dvr_volume_histogram(const float* data, const int64_t size[3], const int64_t strides[3], float* bins,
+                                float* min_density, float* max_density, uint32_t* nonzero);
+
+/* renderer::TFPoint (renderer/tf_utils.h:88-92): position and RGB + opacity. */
+typedef struct dvr_tf_point {
+    float pos;
+    float val[4];
+} dvr_tf_point;
+
+/* sRGB <-> CIE L*a*b* (renderer/tf_utils.h:20-86). */
+void dvr_rgb_to_lab(const float rgb[3], float lab[3]);
+void dvr_lab_to_rgb(const float lab[3], float rgb[3]);
+
+/* TFUtils::assembleFromSettings (renderer/tf_utils.cpp:12-118): merge the editor's colour axis (L*a*b*)
+ * and opacity axis into control points.  *count receives the number of points; points (may be NULL)
+ * receives them when capacity suffices. */
+dvr_status dvr_tf_assemble_from_settings(const float* color_lab, const float* color_pos, int32_t nc,
+                                         const float* opacity, const float* opacity_pos, int32_t no,
+                                         float min_density, float max_density, float opacity_scaling,
+                                         int32_t purge_zero_regions, dvr_tf_point* points, int32_t capacity,
+                                         int32_t* count);
+
+/* TFUtils::getTextureTensor (renderer/tf_utils.cpp:137-172): resolution x 4 texture sampled at the
+ * texel centres. */
+dvr_status dvr_tf_texture_from_points(const dvr_tf_point* points, int32_t n, int32_t resolution, float* texture);
+
 /* ---- camera (replaces renderer/camera_impl.cu:68-93 and :302-357) ------- */
 
 /* Camera::generateRays forward: viewport B x 3 x 3 (eye, right, up) -> per-pixel
