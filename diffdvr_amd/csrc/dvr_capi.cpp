@@ -108,6 +108,7 @@ struct Checked {
     int B, W, H, X, Y, Z;
     int device;
     int filter, camera, tfm, blend;
+    bool preshaded;  // volume holds 4 colour channels in its batch dimension
 };
 
 Checked check_inputs(const dvr_inputs* in)
@@ -127,10 +128,20 @@ Checked check_inputs(const dvr_inputs* in)
         fail(DVR_ERR_UNSUPPORTED,
              "volume is a host (CPU) tensor: this library renders on the GPU only; move the inputs to the HIP device");
     check_dtype(vol, "volume", DVR_FLOAT32);
-    if (in->volume_filter_mode != DVR_FILTER_NEAREST && in->volume_filter_mode != DVR_FILTER_TRILINEAR)
-        fail(DVR_ERR_UNSUPPORTED, "volume_filter_mode %d is not supported (Nearest=0, Trilinear=1)",
+    if (in->volume_filter_mode != DVR_FILTER_NEAREST && in->volume_filter_mode != DVR_FILTER_TRILINEAR &&
+        in->volume_filter_mode != DVR_FILTER_PRESHADED)
+        fail(DVR_ERR_UNSUPPORTED, "volume_filter_mode %d is not supported (Nearest=0, Trilinear=1, Preshaded=3)",
              in->volume_filter_mode);
-    c.B = (int)vol.sizes[0];
+    // renderer.cpp:72-80
+    c.preshaded = in->volume_filter_mode == DVR_FILTER_PRESHADED;
+    if (c.preshaded != (in->tf_mode == DVR_TF_PRESHADED))
+        fail(DVR_ERR_INVALID_ARGUMENT,
+             "if pre-shading is selected, both the filter mode and TF mode must be set to preshaded");
+    if (c.preshaded && vol.sizes[0] != 4)
+        fail(DVR_ERR_INVALID_ARGUMENT,
+             "In pre-shaded mode, the volume colors are encoded in the batch dimension of the volume. It must be 4, "
+             "but is %lld", (long long)vol.sizes[0]);
+    c.B = c.preshaded ? 1 : (int)vol.sizes[0];
     c.X = (int)vol.sizes[1];
     c.Y = (int)vol.sizes[2];
     c.Z = (int)vol.sizes[3];
@@ -242,7 +253,8 @@ Checked check_inputs(const dvr_inputs* in)
         if (tf.sizes[1] <= 1) fail(DVR_ERR_INVALID_ARGUMENT, "tensor 'tf' must have at least two control points in TFLinear-mode");
         break;
     case DVR_TF_GAUSSIAN: check_size(tf, "tf", 2, 6); break;
-    default: fail(DVR_ERR_UNSUPPORTED, "tf mode %d is not supported (Identity=0, Texture=1, Linear=2, Gaussian=3)", c.tfm);
+    case DVR_TF_PRESHADED: break;  // any size, the TF is not read
+    default: fail(DVR_ERR_INVALID_ARGUMENT, "unknown tf enum value (tf mode %d)", c.tfm);
     }
     if (tf.sizes[1] < 1) fail(DVR_ERR_INVALID_ARGUMENT, "tensor 'tf' must have at least one row");
     check_offsets(tf, "tf");
@@ -316,7 +328,8 @@ dvr::AdjointParams check_adjoint(const Checked& c, const dvr_outputs* fwd, const
     if (ao->has_volume_derivatives) {
         const dvr_tensor& t = ao->adj_volume;
         check_dim(t, "adj_volume", 4); check_device(t, "adj_volume", c.device);
-        check_batch(t, "adj_volume", c.B);
+        if (c.preshaded) check_size(t, "adj_volume", 0, 4);  // colour channels (renderer.cpp:354-360)
+        else check_batch(t, "adj_volume", c.B);
         check_size(t, "adj_volume", 1, c.X); check_size(t, "adj_volume", 2, c.Y); check_size(t, "adj_volume", 3, c.Z);
         check_dtype(t, "adj_volume", DVR_FLOAT32); check_offsets(t, "adj_volume");
         check_slice_offsets(t, "adj_volume");
@@ -399,7 +412,12 @@ void* workspace(int device, size_t bytes, hipStream_t stream)
 }
 
 
-bool tiled_ok(const Checked& c) { return (int64_t)c.P.R * c.P.C <= dvr::kMaxTfLds; }
+bool tiled_ok(const Checked& c) { return !c.preshaded && (int64_t)c.P.R * c.P.C <= dvr::kMaxTfLds; }
+
+void no_preshaded(const Checked& c, const char* what)
+{
+    if (c.preshaded) fail(DVR_ERR_UNSUPPORTED, "%s is not available for pre-shaded volumes", what);
+}
 
 // checkpoints per ray: enough for the longest ray when box and step are
 // uniform values, else 1024 samples' worth (longer rays recompute from the
@@ -436,11 +454,13 @@ void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forwar
 {
     Checked c = check_inputs(inputs);
     check_outputs(outputs, c);
-    DeviceGuard g(c.device);
-    hipStream_t s = (hipStream_t)stream;
     int path = opt ? opt->path : DVR_PATH_AUTO;
     const int K = opt ? opt->checkpoint_interval : 0;
     if (path == DVR_PATH_AUTO) path = tiled_ok(c) ? DVR_PATH_TILED : DVR_PATH_PLAIN;
+    if (c.preshaded && (path != DVR_PATH_PLAIN || K > 0))
+        fail(DVR_ERR_UNSUPPORTED, "pre-shaded volumes render on the plain path only");
+    DeviceGuard g(c.device);
+    hipStream_t s = (hipStream_t)stream;
     if (path == DVR_PATH_TILED && !tiled_ok(c))
         fail(DVR_ERR_UNSUPPORTED, "tiled path needs R*C <= %d transfer-function entries", dvr::kMaxTfLds);
     if (K > 0) {
@@ -456,7 +476,9 @@ void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forwar
         if (ck.ndim > 0 && ck.strides[ck.ndim - 1] != 1 && numel > 1)
             fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints must be contiguous");
     }
-    if (path == DVR_PATH_PLAIN) {
+    if (c.preshaded) {
+        check_hip(dvr::launch_forward_preshaded(c.P, c.blend, s), "forward (pre-shaded) launch");
+    } else if (path == DVR_PATH_PLAIN) {
         check_hip(dvr::launch_forward_plain(c.P, c.filter, c.tfm, c.blend, s), "forward (plain) launch");
     } else {
         dvr::CkptParams CK{nullptr, 16, 0, 1};
@@ -478,6 +500,12 @@ void do_adjoint(const dvr_inputs* inputs, const dvr_outputs* fwd, const dvr_tens
     int method = opt ? opt->method : DVR_ADJOINT_AUTO;
     int path = opt ? opt->path : DVR_PATH_AUTO;
     if (method == DVR_ADJOINT_AUTO) method = tiled_ok(c) ? DVR_ADJOINT_CHECKPOINT : DVR_ADJOINT_INVERSION;
+    if (c.preshaded) {
+        if (method != DVR_ADJOINT_INVERSION || path == DVR_PATH_TILED)
+            fail(DVR_ERR_UNSUPPORTED, "pre-shaded volumes use the inversion adjoint on the plain path");
+        check_hip(dvr::launch_adjoint_preshaded(c.P, A, c.blend, s), "adjoint (pre-shaded) launch");
+        return;
+    }
     if (method == DVR_ADJOINT_INVERSION) {
         if (path == DVR_PATH_TILED) fail(DVR_ERR_UNSUPPORTED, "the inversion adjoint runs on the plain path only");
         check_hip(dvr::launch_adjoint_plain(c.P, A, c.filter, c.tfm, c.blend, s), "adjoint (plain) launch");
@@ -512,6 +540,7 @@ void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* lo
               int K, void* stream)
 {
     Checked c = check_inputs(inputs);
+    no_preshaded(c, "fused rendering");
     check_outputs(outputs, c);
     if (!loss) fail(DVR_ERR_INVALID_ARGUMENT, "loss is NULL");
     if (loss->kind != DVR_LOSS_L1 && loss->kind != DVR_LOSS_L2)
@@ -621,6 +650,7 @@ void do_forward_gradients(const dvr_inputs* inputs, const dvr_forward_difference
                           dvr_tensor* gout, void* stream)
 {
     Checked c = check_inputs(inputs);
+    no_preshaded(c, "forward-mode differentiation");
     check_fwd_indices(fd);
     const int D = fd->D;
     // d_tf is read only when TF derivatives are enabled (renderer.cpp:430-447)

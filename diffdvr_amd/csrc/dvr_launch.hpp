@@ -65,6 +65,10 @@ hipError_t launch_adam(const AdamSeg* dSegs, int nSegs, long long maxN, long lon
 hipError_t launch_batched(const BatchParams& P, int filter, int tfm, int blend, int K, int gm, int grid, size_t lds,
                           hipStream_t s, int* occ);
 
+// pre-shaded volumes (dvr_kernels_preshaded.hip): plain structure, inversion adjoint
+hipError_t launch_forward_preshaded(const RenderParams& P, int blend, hipStream_t s);
+hipError_t launch_adjoint_preshaded(const RenderParams& P, const AdjointParams& A, int blend, hipStream_t s);
+
 // forward-mode derivatives (dvr_kernels_fwdgrad.hip)
 hipError_t launch_forward_gradients(const RenderParams& P, const FwdGradParams& G, int filter, int tfm, int blend,
                                     hipStream_t s);

@@ -208,9 +208,6 @@ PackedInputs pack_inputs(const RendererInputs& r)
     in.tf = to_tensor(r.tf);
     in.blend_mode = (int)r.blend_mode;
     in.blending_early_out = r.blending_early_out;
-    if (r.volume_filter_mode == VolumeFilterMode::Preshaded || r.tf_mode == TFMode::Preshaded ||
-        r.tf_mode == TFMode::GaussianLog)
-        throw std::runtime_error("Preshaded / GaussianLog modes are not available in this build");
     return p;
 }
 

@@ -144,13 +144,17 @@ class _RendererFunction(torch.autograd.Function):
         inputs.tf = tf
         inputs.volume = volume
 
-        tensors_for_batches = [camera_matrix, camera_ray_start, camera_ray_dir, tf, volume]
+        tensors_for_batches = [camera_matrix, camera_ray_start, camera_ray_dir, tf]
+        if inputs.volume_filter_mode != pyrenderer.VolumeFilterMode.Preshaded:
+            tensors_for_batches.append(volume)  # a pre-shaded volume holds colour channels instead
         B = max([(1 if t is None else t.shape[0]) for t in tensors_for_batches])
         W, H = inputs.screen_size.x, inputs.screen_size.y
         color = torch.empty(B, H, W, 4, dtype=volume.dtype, device=volume.device)
         term = torch.empty(B, H, W, dtype=torch.int32, device=volume.device)
         outputs = pyrenderer.RendererOutputs(color, term)
-        K = checkpoint_interval if adjoint_method == "checkpoint" else 0
+        # pre-shaded volumes have the reference's plain / inversion path only
+        preshaded = inputs.volume_filter_mode == pyrenderer.VolumeFilterMode.Preshaded
+        K = checkpoint_interval if (adjoint_method == "checkpoint" and not preshaded) else 0
         checkpoints = None
         if K > 0:
             checkpoints = torch.empty(pyrenderer.Renderer.checkpoint_buffer_elements(inputs, K),
@@ -191,7 +195,9 @@ class _RendererFunction(torch.autograd.Function):
         assert implies(optimize_camera, inputs.camera_mode == pyrenderer.CameraMode.RayStartDir)
         _RendererFunction._set_inputs(inputs, camera_matrix, camera_ray_start, camera_ray_dir, tf, volume,
                                       fov_y_radians)
-        tensors_for_batches = [camera_matrix, camera_ray_start, camera_ray_dir, tf, volume]
+        tensors_for_batches = [camera_matrix, camera_ray_start, camera_ray_dir, tf]
+        if inputs.volume_filter_mode != pyrenderer.VolumeFilterMode.Preshaded:
+            tensors_for_batches.append(volume)  # a pre-shaded volume holds colour channels instead
         B = max([(1 if t is None else t.shape[0]) for t in tensors_for_batches])
         W, H = inputs.screen_size.x, inputs.screen_size.y
         color = torch.empty(B, H, W, 4, dtype=volume.dtype, device=volume.device)
@@ -379,6 +385,14 @@ class VolumeDensities(nn.Module):
         return torch.sigmoid(x)
 
 
+class VolumePreshaded(nn.Module):
+    """Pre-shaded colour volume 4*X*Y*Z from an unconstrained input: sigmoid on r, g, b and softplus on
+    the absorption (pytests/diffdvr/parametrizations.py:30-43)."""
+
+    def forward(self, x):
+        return torch.cat([torch.sigmoid(x[:3]), F.softplus(x[3:])], dim=0)
+
+
 class CameraOnASphere(nn.Module):
     """(center B*3, yaw B*1, pitch B*1, distance B*1) -> B*3*3 viewport."""
     ZoomBase = 1.1
@@ -431,6 +445,6 @@ class TfPiecewiseLinear(nn.Module):
         return torch.from_numpy(np.array(tf)).to(dtype=torch.float32).unsqueeze(0)
 
 
-__all__ = ["Renderer", "ProfileRenderer", "Timer", "VolumeDensities", "CameraOnASphere", "TfTexture",
+__all__ = ["Renderer", "ProfileRenderer", "Timer", "VolumeDensities", "VolumePreshaded", "CameraOnASphere", "TfTexture",
            "TfPiecewiseLinear", "fibonacci_sphere", "make_real3", "make_real4", "implies", "toCHW",
            "inverseSigmoid", "inverseSoftplus", "renderer_dtype_torch", "renderer_dtype_np"]

@@ -43,6 +43,7 @@ typedef int32_t dvr_status;
  * pyrenderer enums (bindings/bindings.cpp:215-232). */
 #define DVR_FILTER_NEAREST 0
 #define DVR_FILTER_TRILINEAR 1
+#define DVR_FILTER_PRESHADED 3 /* volume = 4 x X x Y x Z colours (r, g, b, absorption); needs DVR_TF_PRESHADED */
 #define DVR_CAMERA_RAY_START_DIR 0
 #define DVR_CAMERA_INVERSE_VIEW_MATRIX 1
 #define DVR_CAMERA_REFERENCE_FRAME 2
@@ -50,6 +51,8 @@ typedef int32_t dvr_status;
 #define DVR_TF_TEXTURE 1
 #define DVR_TF_LINEAR 2
 #define DVR_TF_GAUSSIAN 3
+#define DVR_TF_PRESHADED 5     /* the TF passes the pre-shaded colour on; any TF tensor size */
+#define DVR_TF_GAUSSIAN_LOG 6  /* log of the Gaussian TF: pre-shading only, not a render mode (as the reference) */
 #define DVR_BLEND_BEER_LAMBERT 0
 #define DVR_BLEND_ALPHA 1
 

@@ -36,9 +36,9 @@
 
 namespace oracle {
 
-enum FilterMode { FilterNearest = 0, FilterTrilinear = 1 };
+enum FilterMode { FilterNearest = 0, FilterTrilinear = 1, FilterPreshaded = 3 };
 enum CameraMode { CameraRayStartDir = 0, CameraInverseViewMatrix = 1, CameraReferenceFrame = 2 };
-enum TFMode { TFIdentity = 0, TFTexture = 1, TFLinear = 2, TFGaussian = 3 };
+enum TFMode { TFIdentity = 0, TFTexture = 1, TFLinear = 2, TFGaussian = 3, TFPreshaded = 5 };
 enum BlendMode { BlendBeerLambert = 0, BlendAlpha = 1 };
 enum AdjointMethod { AdjointInversion = 0, AdjointExactReplay = 1 };
 
@@ -234,6 +234,20 @@ inline T fetch_density(const Inputs<T>& in, int b, V3<T> p)
     TriCell<T> c;
     trilinear_cell(in, vb, p, c);
     return trilinear_value(c);
+}
+
+// Pre-shaded volume (renderer_interpolation.cuh:390-430): the four batch slices
+// of the volume hold r, g, b and absorption; each is interpolated trilinearly
+// and the TF passes the colour on (renderer_tf.cuh:818-854).
+template <class T>
+inline V4<T> preshaded_fetch(const Inputs<T>& in, V3<T> p, TriCell<T> cells[4])
+{
+    T v[4];
+    for (int ch = 0; ch < 4; ++ch) {
+        trilinear_cell(in, ch, p, cells[ch]);
+        v[ch] = trilinear_value(cells[ch]);
+    }
+    return v4(v[0], v[1], v[2], v[3]);
 }
 
 // Trilinear adjoint (:360-387): per-corner weight products; adj_p = adj_cw - adj_fw.
@@ -512,16 +526,22 @@ inline void march_forward(const Inputs<T>& in, int x, int y, int b, int frozen_n
             if (acc.w > in.early_out) break;
         }
         const V3<T> p = sample_pos(r, t);
-        T d;
-        if (in.filter == FilterNearest) {
-            d = fetch_density(in, b, p);
+        V4<T> s;
+        if (in.filter == FilterPreshaded) {
+            TriCell<T> cells[4];
+            s = preshaded_fetch(in, p, cells);
         } else {
-            const int ix = (int)p.x, iy = (int)p.y, iz = (int)p.z;
-            if (ix != cell.ix || iy != cell.iy || iz != cell.iz) trilinear_cell(in, vb, p, cell);
-            cell.fx = p.x - T(ix); cell.fy = p.y - T(iy); cell.fz = p.z - T(iz);
-            d = trilinear_value(cell);
+            T d;
+            if (in.filter == FilterNearest) {
+                d = fetch_density(in, b, p);
+            } else {
+                const int ix = (int)p.x, iy = (int)p.y, iz = (int)p.z;
+                if (ix != cell.ix || iy != cell.iy || iz != cell.iz) trilinear_cell(in, vb, p, cell);
+                cell.fx = p.x - T(ix); cell.fy = p.y - T(iy); cell.fz = p.z - T(iz);
+                d = trilinear_value(cell);
+            }
+            s = tf_eval(in, b, d);
         }
-        const V4<T> s = tf_eval(in, b, d);
         if (a_in_trace) a_in_trace->push_back(acc.w);
         acc = blend(in.blend, acc, s, r.step);
     }
@@ -558,16 +578,22 @@ inline void march_adjoint(const Inputs<T>& in, int x, int y, int b, V4<T> color_
         const T t = r.t0 + T(i) * r.step;
         const V3<T> p = sample_pos(r, t);
         TriCell<T> cell{};
-        T d;
+        TriCell<T> cells[4];
+        T d = T(0);
         int nx = 0, ny = 0, nz = 0;
-        if (in.filter == FilterNearest) {
-            nearest_index(in, p, nx, ny, nz);
-            d = vol_at(in, vb, nx, ny, nz);
+        V4<T> s;
+        if (in.filter == FilterPreshaded) {
+            s = preshaded_fetch(in, p, cells);
         } else {
-            trilinear_cell(in, vb, p, cell);
-            d = trilinear_value(cell);
+            if (in.filter == FilterNearest) {
+                nearest_index(in, p, nx, ny, nz);
+                d = vol_at(in, vb, nx, ny, nz);
+            } else {
+                trilinear_cell(in, vb, p, cell);
+                d = trilinear_value(cell);
+            }
+            s = tf_eval(in, b, d);
         }
-        const V4<T> s = tf_eval(in, b, d);
         T ain;
         if (method == AdjointExactReplay) {
             ain = a_in[(size_t)i];
@@ -577,9 +603,28 @@ inline void march_adjoint(const Inputs<T>& in, int x, int y, int b, V4<T> color_
         }
         V4<T> adj_s; T adj_step_local;
         blend_adjoint_given(in.blend, ain, s, r.step, adj_acc, adj_s, adj_step_local);
-        const T adj_d = tf_adjoint(in, b, d, adj_s, adj_tf);
         V3<T> adj_p = v3(T(0), T(0), T(0));
-        if (in.filter == FilterNearest) {
+        if (in.filter == FilterPreshaded) {
+            // the gradient of each channel's trilinear form (the reference puts the z term of this
+            // sum into y, renderer_interpolation.cuh:557-562)
+            const T as[4] = {adj_s.x, adj_s.y, adj_s.z, adj_s.w};
+            for (int ch = 0; ch < 4; ++ch) {
+                T w[8];
+                adj_p = adj_p + trilinear_adjoint(cells[ch], as[ch], w);
+                if (adj_vol) {
+                    T* av = out.adj_volume + (size_t)ch * in.X * in.Y * in.Z;
+                    for (int k = 0; k < 8; ++k) {
+                        const int ax = (k >> 2) & 1 ? cells[ch].hx : cells[ch].lx;
+                        const int ay = (k >> 1) & 1 ? cells[ch].hy : cells[ch].ly;
+                        const int az = k & 1 ? cells[ch].hz : cells[ch].lz;
+                        atomic_add(&av[((size_t)ax * in.Y + ay) * in.Z + az], w[k]);
+                    }
+                }
+            }
+        }
+        const T adj_d = in.filter == FilterPreshaded ? T(0) : tf_adjoint(in, b, d, adj_s, adj_tf);
+        if (in.filter == FilterPreshaded) {
+        } else if (in.filter == FilterNearest) {
             if (adj_vol) atomic_add(&adj_vol[((size_t)nx * in.Y + ny) * in.Z + nz], adj_d);
         } else {
             T w[8];
@@ -951,15 +996,21 @@ inline void render_forward_gradients(const Inputs<T>& in, const FwdDiffSettings&
 //   linear TF: control points (renderer_tf.cuh:294-313);
 //   alpha blending: the clamp at absorption*step = 1 (renderer_blending.cuh:118-125);
 //   nearest filtering: the rounding edges p = k + 0.5, where the density itself
-//     jumps (renderer_interpolation.cuh:38-50); counted within eps_pos voxels.
+//     jumps (renderer_interpolation.cuh:38-50); counted within eps_pos voxels;
+//   trilinear / pre-shaded cell faces, p = k for k > 0 (renderer_interpolation.cuh:
+//     218-225), where the position derivative jumps by a second difference of
+//     the volume; counted within eps_face voxels when eps_face > 0 (p = 0 is not
+//     a kink: truncation keeps both sides in cell 0).
 // An FP32 evaluation may place such a sample on the other side of the kink
 // than FP64 does; its adjoint then differs by the jump, which no FP32
-// implementation of the reference's mathematics avoids.  (Trilinear cell
-// faces are kinks too, but the jump of the position derivative there is a
-// second difference of the volume, small for smooth volumes.)
+// implementation of the reference's mathematics avoids.  Cell faces matter for
+// the position and step gradients only (the step gradient weighs a sample's
+// position derivative by its index), so they are counted on request.
+// Pre-shaded volumes have no TF kinks; their alpha clamp uses the absorption
+// channel.
 // ---------------------------------------------------------------------------
 template <class T>
-inline int kink_samples(const Inputs<T>& in, int x, int y, int b, int n, T eps, T eps_pos)
+inline int kink_samples(const Inputs<T>& in, int x, int y, int b, int n, T eps, T eps_pos, T eps_face)
 {
     const Ray<T> r = setup_ray(in, x, y, b);
     const T* tf = in.tf + (size_t)bidx(in.tf_b, b) * in.R * in.C;
@@ -970,7 +1021,12 @@ inline int kink_samples(const Inputs<T>& in, int x, int y, int b, int n, T eps, 
         const V3<T> p = sample_pos(r, t);
         // density and the magnitude its FP32 rounding error scales with
         T d, scale;
-        if (in.filter == FilterNearest) {
+        T absorption = T(0);
+        if (in.filter == FilterPreshaded) {
+            TriCell<T> cells[4];
+            absorption = preshaded_fetch(in, p, cells).w;
+            d = scale = T(0);
+        } else if (in.filter == FilterNearest) {
             d = fetch_density(in, b, p);
             scale = std::fabs(d);
         } else {
@@ -991,12 +1047,17 @@ inline int kink_samples(const Inputs<T>& in, int x, int y, int b, int n, T eps, 
             for (int k = 0; k < in.R; ++k) kink |= std::fabs(d - tf[k * in.C + 4]) < band;
         }
         if (in.blend == BlendAlpha) {
-            const T a = tf_eval(in, b, d).w * r.step;
+            const T a = (in.filter == FilterPreshaded ? absorption : tf_eval(in, b, d).w) * r.step;
             kink |= std::fabs(a - T(1)) < eps;
         }
+        const T q[3] = {p.x, p.y, p.z};
         if (in.filter == FilterNearest) {
-            const T q[3] = {p.x, p.y, p.z};
             for (int k = 0; k < 3; ++k) kink |= std::fabs(q[k] - std::floor(q[k]) - T(0.5)) < eps_pos;
+        } else if (eps_face > T(0)) {
+            for (int k = 0; k < 3; ++k) {
+                const T face = std::round(q[k]);
+                kink |= face != T(0) && std::fabs(q[k] - face) < eps_face;
+            }
         }
         count += kink ? 1 : 0;
     }
@@ -1004,7 +1065,7 @@ inline int kink_samples(const Inputs<T>& in, int x, int y, int b, int n, T eps, 
 }
 
 template <class T>
-inline void kink_census(const Inputs<T>& in, const int* term, T eps, T eps_pos, int* out)
+inline void kink_census(const Inputs<T>& in, const int* term, T eps, T eps_pos, T eps_face, int* out)
 {
     const long long count = (long long)in.B * in.H * in.W;
 #pragma omp parallel for schedule(guided)
@@ -1012,7 +1073,7 @@ inline void kink_census(const Inputs<T>& in, const int* term, T eps, T eps_pos, 
         const int b = (int)(i / ((long long)in.W * in.H));
         const int y = (int)((i - (long long)b * in.W * in.H) / in.W);
         const int x = (int)(i - (long long)in.W * (y + (long long)in.H * b));
-        out[i] = kink_samples(in, x, y, b, term[i], eps, eps_pos);
+        out[i] = kink_samples(in, x, y, b, term[i], eps, eps_pos, eps_face);
     }
 }
 

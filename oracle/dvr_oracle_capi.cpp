@@ -79,9 +79,9 @@ template <class T> oracle::Inputs<T> tf_only(int mode, const T* tf, int R, int C
         oracle::render_forward<T>(convert<T>(in), color, term, frozen_term, frozen_t0, t0_out);                \
     }                                                                                                          \
     extern "C" void oracle_kink_census_##SUF(const oracle_inputs* in, const int* term, T eps, T eps_pos,        \
-                                             int* out)                                                         \
+                                             T eps_face, int* out)                                             \
     {                                                                                                          \
-        oracle::kink_census<T>(convert<T>(in), term, eps, eps_pos, out);                                       \
+        oracle::kink_census<T>(convert<T>(in), term, eps, eps_pos, eps_face, out);                             \
     }                                                                                                          \
     /* forward-mode derivatives: grad is B*H*W*D*4; d_start/d_dir hold 3 indices; d_tf may be null */          \
     extern "C" void oracle_render_forward_gradients_##SUF(const oracle_inputs* in, int D, int d_step,            \

@@ -94,7 +94,8 @@ def _ptr(a):
 
 def batch_size(scene) -> int:
     B = 1
-    keys = ["volume", "box_min", "box_size", "step", "tf"]
+    # a pre-shaded volume (filter 3) holds colour channels in its batch dimension
+    keys = (["volume"] if int(scene["filter"]) != 3 else []) + ["box_min", "box_size", "step", "tf"]
     keys += ["ray_start", "ray_dir"] if scene["camera"] == 0 else ["cam_matrix"]
     for k in keys:
         b = scene[k].shape[0]
@@ -206,16 +207,18 @@ def adjoint(scene, color, term, adj_color, grads=("tf",), dtype=np.float64, meth
     return out
 
 
-def kink_census(scene, term, eps=1e-5, eps_pos=2e-5):
+def kink_census(scene, term, eps=1e-5, eps_pos=2e-5, eps_face=0.0):
     """Per-ray count (B,H,W) of samples near a kink of the reference's mathematics,
     evaluated in FP64 with the given sample counts: within eps * (cell magnitude)
-    of a TF clamp / texel edge / control point or of the alpha clamp, or within
-    eps_pos voxels of a nearest-neighbour rounding edge."""
+    of a TF clamp / texel edge / control point or of the alpha clamp, within
+    eps_pos voxels of a nearest-neighbour rounding edge, or (eps_face > 0) within
+    eps_face voxels of an interior trilinear cell face, where the position
+    derivative jumps."""
     p = _Prepared(scene, np.float64)
     t = np.ascontiguousarray(term, dtype=np.int32)
     out = np.zeros((p.B, p.H, p.W), dtype=np.int32)
     lib().oracle_kink_census_f64(ctypes.byref(p.s), _ptr(t), ctypes.c_double(eps), ctypes.c_double(eps_pos),
-                                 _ptr(out))
+                                 ctypes.c_double(eps_face), _ptr(out))
     return out
 
 

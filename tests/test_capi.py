@@ -115,7 +115,10 @@ def test_host_tensors_are_rejected_loudly():
     (lambda i: setattr(i, "camera_matrix", tensor((2, 3, 3), 0)) or setattr(i, "tf", tensor((3, 4, 4), 0)), 1,
      b"must be have the same batch size"),
     (lambda i: setattr(i, "camera_mode", 7), 1, b"unknown camera enum value"),
-    (lambda i: setattr(i, "tf_mode", 6), 3, b"tf mode 6 is not supported"),
+    (lambda i: setattr(i, "tf_mode", 6), 1, b"unknown tf enum value"),
+    (lambda i: setattr(i, "tf_mode", 5), 1, b"both the filter mode and TF mode must be set to preshaded"),
+    (lambda i: setattr(i, "volume_filter_mode", 3) or setattr(i, "tf_mode", 5), 1,
+     b"volume colors are encoded in the batch dimension of the volume. It must be 4"),
     (lambda i: setattr(i, "step_size_value", 0.0), 1, b"step size must be positive"),
     (lambda i: setattr(i, "volume", tensor((1, 1, 5, 5), 0)), 1, b"at least 2 voxels"),
 ])
@@ -127,6 +130,24 @@ def test_validation_messages(mutate, code, msg):
     st = l.dvr_render_forward(ctypes.byref(i), ctypes.byref(o), None)
     assert st == code, (st, l.dvr_last_error())
     assert msg in l.dvr_last_error()
+
+
+class FwdOptions(ctypes.Structure):
+    _fields_ = [("path", ctypes.c_int32), ("checkpoint_interval", ctypes.c_int32), ("checkpoints", Tensor)]
+
+
+@pytest.mark.parametrize("path,K", [(2, 0), (0, 16)])
+def test_preshaded_renders_on_the_plain_path_only(path, K):
+    """A pre-shaded volume (4 colour channels) with the tiled path or checkpoints asked for is
+    refused before any HIP call, naming the plain path."""
+    l = lib()
+    i = valid_inputs(0)
+    i.volume_filter_mode, i.tf_mode = 3, 5
+    i.volume = tensor((4, 5, 5, 5), 0)
+    o = Outputs(tensor((1, 6, 8, 4), 0), tensor((1, 6, 8), 0, dtype=1))
+    opt = FwdOptions(path, K, tensor((1 << 20,), 0))
+    assert l.dvr_render_forward_ex(ctypes.byref(i), ctypes.byref(o), ctypes.byref(opt), None) == 3
+    assert b"pre-shaded volumes render on the plain path only" in l.dvr_last_error()
 
 
 def test_output_shape_mismatch_and_null_arguments():

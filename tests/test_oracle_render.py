@@ -16,7 +16,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import oracle, camera_scene
+from helpers import oracle, camera_scene, to_f32
 
 
 def axis_scene(density, absorption, color, step, blend=0, W=4, H=3, size=(6, 5, 7)):
@@ -262,3 +262,46 @@ def test_kink_census():
     # p.z = 0.75, 2.25, 3.75, 5.25 never do
     s5 = dict(s, filter=0)
     assert np.all(oracle.kink_census(s5, t) == 5)
+    assert np.all(oracle.kink_census(s5, t, eps_face=1e-6) == 5)  # faces are a trilinear kink only
+    # trilinear cell faces on request: p.x = 1, ..., 5 on every ray (p.x = 0 is not a kink), and every
+    # sample of the middle row, whose p.y = 2 lies on a face
+    census = oracle.kink_census(s, t, eps_face=1e-6)
+    assert np.all(census[0, [0, 2]] == 5) and np.all(census[0, 1] == t[0, 1])
+    # pre-shaded volume: the alpha clamp reads the absorption channel, 10 * 0.1 = 1
+    s6 = dict(s, filter=3, tf_mode=5, blend=1,
+              volume=np.broadcast_to(np.array([0.6, 0.6, 0.6, 10.0])[:, None, None, None], (4, 6, 5, 7)).copy())
+    assert np.all(oracle.kink_census(s6, t) == t)
+    s6["volume"][3] = 9.0
+    assert np.all(oracle.kink_census(s6, t) == 0)
+
+
+def preshaded_scene(rng, W=6, H=5, blend=0):
+    """A pre-shaded colour volume (r, g, b, absorption in the 4 batch slices) seen through per-pixel rays."""
+    s = camera_scene(rng, W, H, filter=1, tf_mode=1, blend=blend, camera=0, smooth=True, X=12, Y=10, Z=14)
+    vol = s["volume"][0]
+    s["volume"] = to_f32(np.stack([vol * 0.9, 1.0 - vol, 0.5 + 0.4 * vol, vol * (4.0 if blend == 0 else 2.0)]))
+    s["filter"], s["tf_mode"] = 3, 5
+    return s
+
+
+@pytest.mark.parametrize("blend", [0, 1])
+def test_render_fd_preshaded(blend):
+    """Pre-shaded volumes (renderer_interpolation.cuh:390-579): FP64 central differences vs both oracle
+    adjoints for the 4-channel volume, per-pixel rays and step."""
+    rng = np.random.default_rng(300 + blend)
+    s = preshaded_scene(rng, blend=blend)
+    s["step"] = np.full((1, s["H"], s["W"]), 0.03) + rng.uniform(-0.005, 0.005, (1, s["H"], s["W"]))
+    assert oracle.batch_size(s) == 1
+    fd_check(s, ("volume", "camera", "step"), rng)
+
+
+def test_preshaded_constant_colour_closed_form():
+    """A constant pre-shaded volume (r, g, b, a) under Beer-Lambert: colour = c * (1 - e^{-a dt n})."""
+    s = preshaded_scene(np.random.default_rng(302))
+    c = np.array([0.2, 0.5, 0.7, 3.0])
+    s["volume"] = np.broadcast_to(c[:, None, None, None], s["volume"].shape).copy()
+    col, term = oracle.forward(s)
+    dt = float(s["step"].reshape(-1)[0])
+    A = 1.0 - np.exp(-3.0 * dt * term)
+    assert np.allclose(col[..., 3], A, atol=1e-12)
+    assert np.allclose(col[..., :3], A[..., None] * c[None, None, None, :3], atol=1e-12)
