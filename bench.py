@@ -19,9 +19,11 @@ Workloads (synthetic, seeded, identical on every host; diffdvr_amd/scenes.py):
      volume, L2, density + TF gradient, HIP Adam, 100 steps captured as one HIP graph
   fwd: forward mode on the C2 scene, D = 7 (per-pixel step size, ray start, ray
      direction), contracted with the L1 loss gradient
+  preshade: TF pre-shading of a 256^3 volume (4-Gaussian TF) and its density
+     adjoint, with best fit, log-MSE and importance sampling timed beside it
 
 Prints ONE JSON line.  Options:
-  --workload c2|c3|c4|c5|fwd  --steps K --warmup W  --gpus N (only 1)
+  --workload c2|c3|c4|c5|fwd|preshade  --steps K --warmup W  --gpus N (only 1)
   --no-cpu-baseline  --profile (short run for rocprofv3: no plain leg, no baseline)
   --path persistent|fused (c2: persistent plan, or the single-batch fused kernel)
   --checkpoint-interval K  --wave-width W  --grid G  (all 0 = autotuned; c4 uses defaults)
@@ -51,7 +53,7 @@ METRIC = "fwd+adjoint ray-samples/s on 1 MI355X, batched vs plain, at 1e-4 rel g
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "fwd"])
+    p.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "fwd", "preshade"])
     p.add_argument("--steps", type=int, default=0, help="timed steps (0 = per-workload default)")
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -672,13 +674,151 @@ def bench_fwd(args):
     return result
 
 
+# ---------------------------------------------------------------------------
+# pre-shading step of the density-from-colour reconstruction
+# (pytests/tests/volume/preshaded_to_density.py:110-140): colour = TF(density),
+# weighted MSE against a target colour volume, backward to the density
+# ---------------------------------------------------------------------------
+PRESHADE_BYTES = {"forward": 20, "adjoint": 24}   # per voxel: 4 B density (+16 B adj colour) in, 16 B colour / 4 B out
+
+
+def graph_of(fn, reps):
+    """fn called reps times, captured as one HIP graph (fn launches no memset)."""
+    import torch
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fn()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(reps):
+            fn()
+    torch.cuda.synchronize()
+    return g
+
+
+def cpu_baseline_preshade(dens_cpu, tf_cpu):
+    import numpy as np
+    from oracle import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    d = dens_cpu[0].numpy().astype(np.float32)
+    tf = tf_cpu[0].numpy().astype(np.float32)
+    adj = np.ones((4,) + d.shape, dtype=np.float32)
+    t0 = time.perf_counter()
+    oracle.preshade(3, tf, d, dtype=np.float32)
+    oracle.preshade_adjoint(3, tf, d, adj, dtype=np.float32)
+    dt = time.perf_counter() - t0
+    return {"value": d.size / dt, "unit": "voxels/s", "cores": threads, "kind": "port",
+            "sample": f"the whole {d.shape[0]}^3 volume ({dt:.2f} s): FP32 pre-shading and its adjoint with the "
+                      f"Gaussian TF, the restatement of renderer/tf_utils_impl.cu:17-121 (OpenMP)"}
+
+
+def bench_preshade(args):
+    import numpy as np
+    import torch
+    import torch.nn.functional as F
+    from diffdvr_amd import pyrenderer as pr
+    from diffdvr_amd import scenes
+    from diffdvr_amd._native import native as n
+
+    dev = torch.device("cuda:0")
+    N = 256
+    steps = args.steps or 20
+    args.steps = steps
+    dens_cpu = scenes.blob_volume(N, 12, seed=41)
+    tf_cpu = scenes.gaussian_tf(4, seed=42)
+    dens = dens_cpu.to(dev)
+    tf = tf_cpu.to(dev)
+    target = pr.TFUtils.preshade_volume(scenes.blob_volume(N, 12, seed=43).to(dev), tf, pr.TFMode.Gaussian)
+    weight = 0.5 + 0.5 * torch.clamp(target[3:4], 0, 1)
+    param = dens.clone().requires_grad_(True)
+    stream = torch.cuda.current_stream()
+    color = torch.empty(4, N, N, N, device=dev)
+    adj_color = torch.rand(4, N, N, N, device=dev)
+    adj_dens = torch.empty_like(dens)
+
+    def fwd():
+        n.preshade_volume(dens, tf, 3, color)
+
+    def adj():
+        n.preshade_volume_adjoint(dens, tf, 3, adj_color, adj_dens)
+
+    def train_step():
+        param.grad = None
+        c = pr.TFUtils.preshade_volume(param, tf, pr.TFMode.Gaussian)
+        loss = torch.mean(F.mse_loss(c, target, reduction="none") * weight)
+        loss.backward()
+
+    if args.profile:  # 3 launches of each kernel and one 256 MiB device copy (counter calibration)
+        for _ in range(3):
+            fwd()
+            adj()
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": True, "voxels": N ** 3, "copy_bytes": src.numel() * 4}))
+        return None
+    for _ in range(args.warmup):
+        fwd()
+        adj()
+        train_step()
+    torch.cuda.synchronize()
+    # kernel durations: 10 launches captured in a HIP graph, replayed back to back (an eager
+    # Python loop leaves the GPU idle between launches of a 60 us kernel)
+    reps = 10
+    fwd_ms = timed(graph_of(fwd, reps).replay, steps, stream) / reps
+    adj_ms = timed(graph_of(adj, reps).replay, steps, stream) / reps
+    eager_ms = timed(lambda: (fwd(), adj()), steps, stream)
+    step_ms = timed(train_step, steps, stream)
+    voxels = float(N ** 3)
+    # best fit at the reference script's default of 256 candidates, on 128^3
+    c128 = pr.TFUtils.preshade_volume(scenes.blob_volume(128, 12, seed=44).to(dev), tf, pr.TFMode.Gaussian)
+    fit_ms = timed(lambda: pr.TFUtils.find_best_fit(c128, tf, pr.TFMode.Gaussian, 256, 1.0), 3, stream)
+    # elementwise log-MSE over the colour volume
+    lx, ly = torch.log(color + 1e-3), torch.log(target + 1e-3)
+    lmse_out = torch.empty_like(lx)
+    lmse_ms = timed(graph_of(lambda: n.log_loss(False, lx, ly, lmse_out), reps).replay, steps, stream) / reps
+    # importance sampling of 2^22 positions
+    locs = torch.rand(1 << 22, 3, device=dev)
+    imp_ms = timed(lambda: pr.sample_importance(locs, dens, tf, pr.TFMode.Gaussian, 1, 1, 1, 1, 7), 5, stream)
+    achieved = voxels * PRESHADE_BYTES["forward"] / (fwd_ms * 1e-3)
+    value = voxels / ((fwd_ms + adj_ms) * 1e-3)
+    config = {"workload": "pre-shading of a 256^3 blob density volume through a 4-Gaussian TF and its density "
+                          "adjoint (the loss step of preshaded_to_density.py); extras: the full torch loss step, "
+                          "best fit (256 candidates, 128^3), log-MSE over 4x256^3, importance sampling of 2^22 "
+                          "positions",
+              "voxels": N ** 3, "tf": "Gaussian, R = 4"}
+    r = {"metric": "pre-shading forward+adjoint voxels/s on 1 MI355X", "value": value, "unit": "voxels/s",
+         "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": fwd_ms + adj_ms,
+         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+         "config": config,
+         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                      "frac": achieved / HBM_PEAK, "traffic": traffic_from_profiles("pmc_traffic_preshade_256.txt"),
+                      "kernel": "dvr::k_preshade<3, false>", "kernel_ms": fwd_ms,
+                      "bytes_per_voxel": PRESHADE_BYTES["forward"]},
+         "adjoint_ms": adj_ms, "eager_python_fwd_adj_ms": eager_ms,
+         "adjoint_GBps": voxels * PRESHADE_BYTES["adjoint"] / (adj_ms * 1e-3) / 1e9,
+         "torch_loss_step_ms": step_ms,
+         "best_fit_candidates_per_s": 128 ** 3 * 256 / (fit_ms * 1e-3),
+         "log_mse_GBps": lx.numel() * 12 / (lmse_ms * 1e-3) / 1e9,
+         "importance_samples_per_s": (1 << 22) / (imp_ms * 1e-3)}
+    if not args.no_cpu_baseline:
+        r["cpu_baseline"] = cpu_baseline_preshade(dens_cpu, tf_cpu)
+    return r
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported; this build measures one MI355X "
               f"(multi-GPU sharding is a design note, not a build target)", file=sys.stderr)
         sys.exit(2)
-    fn = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "fwd": bench_fwd}[args.workload]
+    fn = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "fwd": bench_fwd,
+          "preshade": bench_preshade}[args.workload]
     result = fn(args)
     if result is not None:
         print(json.dumps(result))

@@ -35,33 +35,12 @@ using dvr_host::check_hip;
 using dvr_host::guarded;
 using dvr_host::DeviceGuard;
 
-const char* dtype_name(int d) { return d == DVR_FLOAT32 ? "torch.float32" : (d == DVR_INT32 ? "torch.int32" : "?"); }
+using dvr_host::check_device;
+using dvr_host::check_dim;
+using dvr_host::check_dtype;
+using dvr_host::check_size;
+using dvr_host::given;
 
-bool given(const dvr_tensor& t) { return t.data != nullptr || t.ndim > 0; }
-
-// CHECK_DIM, CHECK_SIZE, CHECK_DTYPE, CHECK_CUDA of renderer/pytorch_utils.h:5-11
-void check_dim(const dvr_tensor& t, const char* name, int d)
-{
-    if (!given(t)) fail(DVR_ERR_INVALID_ARGUMENT, "%s is not defined!", name);
-    if (t.ndim != d) fail(DVR_ERR_INVALID_ARGUMENT, "%s must be a tensor with %d dimensions, but has %d", name, d, t.ndim);
-}
-void check_size(const dvr_tensor& t, const char* name, int d, int64_t s)
-{
-    if (t.sizes[d] != s)
-        fail(DVR_ERR_INVALID_ARGUMENT, "%s must have %lld entries at dimension %d, but has %lld entries", name,
-             (long long)s, d, (long long)t.sizes[d]);
-}
-void check_dtype(const dvr_tensor& t, const char* name, int dt)
-{
-    if (t.dtype != dt)
-        fail(DVR_ERR_INVALID_ARGUMENT, "%s must be of type %s, but is %s", name, dtype_name(dt), dtype_name(t.dtype));
-}
-void check_device(const dvr_tensor& t, const char* name, int device)
-{
-    if (t.device != device)
-        fail(DVR_ERR_INVALID_ARGUMENT, "All tensors must be on the same device, but %s is on device %d (expected %d)",
-             name, t.device, device);
-}
 // checkBatch: size 1 broadcasts
 int check_batch(const dvr_tensor& t, const char* name, int B1, int idx = 0)
 {

@@ -49,6 +49,43 @@ template <class F> dvr_status guarded(F&& f)
     }
 }
 
+inline const char* dtype_name(int d)
+{
+    switch (d) {
+    case DVR_FLOAT32: return "torch.float32";
+    case DVR_INT32: return "torch.int32";
+    case DVR_FLOAT64: return "torch.float64";
+    case DVR_BOOL: return "torch.bool";
+    default: return "?";
+    }
+}
+
+inline bool given(const dvr_tensor& t) { return t.data != nullptr || t.ndim > 0; }
+
+// CHECK_DIM, CHECK_SIZE, CHECK_DTYPE, CHECK_CUDA of renderer/pytorch_utils.h:5-11
+inline void check_dim(const dvr_tensor& t, const char* name, int d)
+{
+    if (!given(t)) fail(DVR_ERR_INVALID_ARGUMENT, "%s is not defined!", name);
+    if (t.ndim != d) fail(DVR_ERR_INVALID_ARGUMENT, "%s must be a tensor with %d dimensions, but has %d", name, d, t.ndim);
+}
+inline void check_size(const dvr_tensor& t, const char* name, int d, int64_t s)
+{
+    if (t.sizes[d] != s)
+        fail(DVR_ERR_INVALID_ARGUMENT, "%s must have %lld entries at dimension %d, but has %lld entries", name,
+             (long long)s, d, (long long)t.sizes[d]);
+}
+inline void check_dtype(const dvr_tensor& t, const char* name, int dt)
+{
+    if (t.dtype != dt)
+        fail(DVR_ERR_INVALID_ARGUMENT, "%s must be of type %s, but is %s", name, dtype_name(dt), dtype_name(t.dtype));
+}
+inline void check_device(const dvr_tensor& t, const char* name, int device)
+{
+    if (t.device != device)
+        fail(DVR_ERR_INVALID_ARGUMENT, "All tensors must be on the same device, but %s is on device %d (expected %d)",
+             name, t.device, device);
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)

@@ -76,4 +76,14 @@ hipError_t launch_forward_to_adjoint(const FwdToAdjParams& F, const int* dtf, in
                                      int tSb, int tSr, int tSc, int R, int C, hipStream_t s);
 hipError_t launch_compare_to_image(const CompareParams& Q, hipStream_t s);
 
+// TF utilities (dvr_kernels_tfutil.hip).  tfm 0..3 or 6 (GaussianLog, pre-shading only).
+hipError_t launch_preshade(const PreshadeParams& P, int tfm, bool adjoint, hipStream_t s);
+hipError_t launch_best_fit(const BestFitParams& P, int tfm, hipStream_t s);
+// maxima pass (clears P.maxima first), then one decision per sample
+hipError_t launch_sample_importance(const ImportanceParams& P, int tfm, hipStream_t s);
+
+// elementwise losses (dvr_kernels_losses.hip).  op: 0 mul_log, 1 mul_log backward,
+// 2 log_mse, 3 log_mse backward, 4 log_l1, 5 log_l1 backward; dbl: float64 data.
+hipError_t launch_elementwise(const ElementwiseParams& P, int op, bool dbl, hipStream_t s);
+
 }  // namespace dvr

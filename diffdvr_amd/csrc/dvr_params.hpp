@@ -194,4 +194,81 @@ struct CompareParams {
     float norm;                            // 1 / (B*H*W)
 };
 
+// ---------------------------------------------------------------------------
+// Voxel-wise kernels over an X x Y x Z grid (pre-shading, best fit,
+// importance maxima).  A flat index i runs over loop levels n[0] (innermost),
+// n[1], n[2]; level k is the logical axis axis[k] (0 = x, 1 = y, 2 = z).  The
+// host puts the axis of the main input's smallest stride innermost, so
+// neighbouring lanes touch neighbouring elements.  Divisions by n[0] and n[1]
+// use multiply-high magic numbers (Granlund and Montgomery, PLDI 1994):
+// q = (umulhi(n, m) + n) >> l for n < 2^31.
+// ---------------------------------------------------------------------------
+constexpr int kMaxAuxTf = 16384;  // max R*C floats of a TF staged in LDS by the voxel-wise kernels
+
+struct FastDiv {
+    unsigned m;
+    int l;
+};
+
+struct VoxelGrid {
+    int X, Y, Z;
+    int n[3];
+    int axis[3];
+    FastDiv div0, div1;
+    int total;
+};
+
+// a TF table (one batch entry) in global memory: row stride sr, column stride sc
+struct TfTable {
+    const float* p;
+    int R, C, sr, sc;
+};
+
+// per-tensor element strides for the logical axes x, y, z (+ channel)
+struct Strides3 {
+    int x, y, z, c;
+};
+
+// TFUtils.preshade_volume and its adjoint (renderer/tf_utils_impl.cu:17-190)
+struct PreshadeParams {
+    VoxelGrid G;
+    TfTable tf;
+    const float* dens; Strides3 ds;   // 1 x X x Y x Z
+    float* col; Strides3 cs;          // 4 x X x Y x Z (forward output)
+    const float* acol; Strides3 as;   // 4 x X x Y x Z (adjoint input)
+    float* adens; Strides3 ads;       // 1 x X x Y x Z (adjoint output, overwritten)
+};
+
+// TFUtils.find_best_fit (renderer/tf_utils_impl.cu:225-520)
+struct BestFitParams {
+    VoxelGrid G;
+    TfTable tf;
+    const float* col; Strides3 cs;    // 4 x X x Y x Z target colours
+    const float* prev; Strides3 ps;   // previous densities or NULL (no neighbour term)
+    float* out; Strides3 os;          // 1 x X x Y x Z
+    int numSamples;
+    float opacityWeighting, neighborWeighting;
+    unsigned long long seed;
+};
+
+// sample_importance (renderer/losses_impl.cu:254-421)
+struct ImportanceParams {
+    VoxelGrid G;                       // maxima pass
+    TfTable tf;
+    const float* dens; Strides3 ds;   // 1 x X x Y x Z
+    const float* loc; int lsN, lsC;   // N x 3 sample locations in [0,1]^3
+    unsigned char* out; int os;       // N booleans
+    int N;
+    float* maxima;                     // 3 floats: max |grad density|, max opacity, max |grad opacity|
+    float wUniform, wDensityGradient, wOpacity;  // cumulative, normalised weights
+    unsigned long long seed;
+};
+
+// elementwise losses (renderer/losses_impl.cu:35-240): contiguous arrays of n elements
+struct ElementwiseParams {
+    const void* a; const void* b; const void* g;
+    void* o1; void* o2;
+    long long n;
+};
+
 }  // namespace dvr

@@ -84,7 +84,7 @@ struct ForwardDifferencesSettings {
 // tensor -> dvr_tensor
 // ---------------------------------------------------------------------------
 struct TorchInfo {
-    py::object torch, cuda_current_stream, cuda_current_device, float32, int32;
+    py::object torch, cuda_current_stream, cuda_current_device, float32, int32, float64, bool_;
 };
 TorchInfo& torch_info()
 {
@@ -96,6 +96,8 @@ TorchInfo& torch_info()
         t->cuda_current_device = t->torch.attr("cuda").attr("current_device");
         t->float32 = t->torch.attr("float32");
         t->int32 = t->torch.attr("int32");
+        t->float64 = t->torch.attr("float64");
+        t->bool_ = t->torch.attr("bool");
     }
     return *t;
 }
@@ -118,7 +120,11 @@ dvr_tensor to_tensor(const py::handle& h)
     }
     t.data = reinterpret_cast<void*>(h.attr("data_ptr")().cast<uintptr_t>());
     py::object dt = h.attr("dtype");
-    t.dtype = dt.is(ti.float32) ? DVR_FLOAT32 : (dt.is(ti.int32) ? DVR_INT32 : -1);
+    t.dtype = dt.is(ti.float32) ? DVR_FLOAT32
+              : dt.is(ti.int32)   ? DVR_INT32
+              : dt.is(ti.float64) ? DVR_FLOAT64
+              : dt.is(ti.bool_)   ? DVR_BOOL
+                                  : -1;
     py::object dev = h.attr("device");
     const std::string type = py::str(dev.attr("type"));
     if (type == "cuda") {
@@ -599,6 +605,48 @@ PYBIND11_MODULE(_pyrenderer, m)
           },
           py::arg("color_input"), py::arg("gradients_input"), py::arg("color_reference"), py::arg("difference_out"),
           py::arg("gradients_out"), py::arg("reduce"));
+    // TF utilities, importance sampling and elementwise losses on device tensors
+    m.def("preshade_volume", [](py::object density, py::object tf, int tf_mode, py::object color) {
+        dvr_tensor d = to_tensor(density), t = to_tensor(tf), c = to_tensor(color);
+        check(dvr_preshade_volume(&d, &t, tf_mode, &c, current_stream(d.device)));
+    });
+    m.def("preshade_volume_adjoint",
+          [](py::object density, py::object tf, int tf_mode, py::object adj_color, py::object adj_density) {
+              dvr_tensor d = to_tensor(density), t = to_tensor(tf), a = to_tensor(adj_color);
+              dvr_tensor o = to_tensor(adj_density);
+              check(dvr_preshade_volume_adjoint(&d, &t, tf_mode, &a, &o, current_stream(d.device)));
+          });
+    m.def("find_best_fit", [](py::object color_volume, py::object tf, int tf_mode, int num_samples,
+                              float opacity_weighting, py::object previous, float neighbor_weighting, uint64_t seed,
+                              py::object density) {
+        dvr_tensor c = to_tensor(color_volume), t = to_tensor(tf), p = to_tensor(previous), d = to_tensor(density);
+        check(dvr_find_best_fit(&c, &t, tf_mode, num_samples, opacity_weighting, &p, neighbor_weighting, seed, &d,
+                                current_stream(c.device)));
+    });
+    m.def("sample_importance", [](py::object locations, py::object density, py::object tf, int tf_mode,
+                                  std::vector<float> weights, int seed, py::object maxima, py::object out) {
+        if (weights.size() != 4) throw std::invalid_argument("4 weights expected");
+        dvr_tensor l = to_tensor(locations), d = to_tensor(density), t = to_tensor(tf);
+        dvr_tensor mx = to_tensor(maxima), o = to_tensor(out);
+        check(dvr_sample_importance(&l, &d, &t, tf_mode, weights.data(), seed, &mx, &o, current_stream(d.device)));
+    });
+    m.def("mul_log", [](py::object t, py::object out) {
+        dvr_tensor a = to_tensor(t), o = to_tensor(out);
+        check(dvr_mul_log(&a, &o, current_stream(a.device)));
+    });
+    m.def("mul_log_backward", [](py::object t, py::object grad, py::object grad_t) {
+        dvr_tensor a = to_tensor(t), g = to_tensor(grad), o = to_tensor(grad_t);
+        check(dvr_mul_log_backward(&a, &g, &o, current_stream(a.device)));
+    });
+    m.def("log_loss", [](bool l1, py::object x, py::object y, py::object out) {
+        dvr_tensor a = to_tensor(x), b = to_tensor(y), o = to_tensor(out);
+        check(l1 ? dvr_log_l1(&a, &b, &o, current_stream(a.device)) : dvr_log_mse(&a, &b, &o, current_stream(a.device)));
+    });
+    m.def("log_loss_backward", [](bool l1, py::object x, py::object y, py::object grad, py::object gx, py::object gy) {
+        dvr_tensor a = to_tensor(x), b = to_tensor(y), g = to_tensor(grad), ox = to_tensor(gx), oy = to_tensor(gy);
+        void* s = current_stream(a.device);
+        check(l1 ? dvr_log_l1_backward(&a, &b, &g, &ox, &oy, s) : dvr_log_mse_backward(&a, &b, &g, &ox, &oy, s));
+    });
     // volume files, synthetic volumes, histogram and TF builders (host buffers by address)
     m.def("cvol_read_header", [](const std::string& path) {
         dvr_cvol_header h;

@@ -14,7 +14,9 @@ Extensions (keyword-only, defaults keep the reference behaviour):
 
 ``Volume`` (.cvol files, implicit datasets, mipmaps, histogram) and ``TFUtils``
 (TF editor points, piecewise / texture tensors) read and build host data through
-the C-ABI.
+the C-ABI; ``TFUtils.preshade_volume`` / ``find_best_fit``, ``sample_importance``
+and the losses ``mul_log`` / ``log_mse`` / ``log_l1`` run HIP kernels on device
+tensors (the log losses also have their backward, which the reference raises on).
 
 There is no CPU rendering path: every render call needs HIP device tensors
 and raises ``RuntimeError`` otherwise.
@@ -396,7 +398,51 @@ class TFPoint:
         return f"TFPoint(pos={self.pos}, val=({self.val.x}, {self.val.y}, {self.val.z}, {self.val.w}))"
 
 
+class _PreshadeVolume(torch.autograd.Function):
+    """renderer/tf_utils.cpp:173-247: color (4 x X x Y x Z) = TF(density); backward: the density adjoint."""
+
+    @staticmethod
+    def forward(ctx, density, tf, tf_mode):
+        out = torch.empty((4,) + tuple(density.shape[1:]), dtype=density.dtype, device=density.device)
+        _n.preshade_volume(density, tf, int(tf_mode), out)
+        ctx.save_for_backward(density, tf)
+        ctx.tf_mode = int(tf_mode)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        density, tf = ctx.saved_tensors
+        adj = torch.empty_like(density)
+        _n.preshade_volume_adjoint(density, tf, ctx.tf_mode, grad_output, adj)
+        return adj, None, None
+
+
 class TFUtils:
+    @staticmethod
+    def preshade_volume(density: torch.Tensor, tf: torch.Tensor, tfMode) -> torch.Tensor:
+        """Pre-shades a density volume (1 x X x Y x Z) through the TF into a colour volume
+        (4 x X x Y x Z: r, g, b, opacity).  Differentiable w.r.t. the density
+        (renderer/tf_utils.cpp:249-253, bindings/bindings.cpp:570-581).  tfMode: Identity,
+        Texture, Linear, Gaussian or GaussianLog."""
+        return _PreshadeVolume.apply(density, tf, tfMode)
+
+    @staticmethod
+    def find_best_fit(color_volume: torch.Tensor, tf: torch.Tensor, tf_mode, num_samples: int,
+                      opacity_weighting: float, previous_densities: torch.Tensor | None = None,
+                      neighbor_weighting: float | None = None) -> torch.Tensor:
+        """The density (1 x X x Y x Z) whose TF colour best matches each voxel of the colour
+        volume among num_samples random densities; with previous_densities, neighbour
+        differences are penalised by neighbor_weighting (renderer/tf_utils.cpp:255-332,
+        bindings/bindings.cpp:582-633).  The seed comes from torch's CPU generator."""
+        if (previous_densities is None) != (neighbor_weighting is None):
+            raise TypeError("previous_densities and neighbor_weighting are given together")
+        out = torch.zeros((1,) + tuple(color_volume.shape[1:]), dtype=color_volume.dtype,
+                          device=color_volume.device)
+        seed = int(torch.randint(0, 2 ** 62, (1,), dtype=torch.int64).item())
+        _n.find_best_fit(color_volume, tf, int(tf_mode), int(num_samples), float(opacity_weighting),
+                         previous_densities, float(neighbor_weighting or 0.0), seed, out)
+        return out
+
     @staticmethod
     def assemble_from_settings(color_values_lab, color_positions, opacity_values, opacity_positions,
                                min_density: float, max_density: float, opacity_scaling: float,
@@ -436,9 +482,86 @@ class TFUtils:
         return real3(*_n.lab_to_rgb(float(lab.x), float(lab.y), float(lab.z)))
 
 
+class _MulLog(torch.autograd.Function):
+    """renderer/losses.cpp:8-48: t log2 t (0 below FLT_EPSILON)."""
+
+    @staticmethod
+    def forward(ctx, t):
+        t = t.contiguous()
+        out = torch.empty_like(t)
+        _n.mul_log(t, out)
+        ctx.save_for_backward(t)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        t, = ctx.saved_tensors
+        grad = torch.empty_like(t)
+        _n.mul_log_backward(t, grad_output.contiguous(), grad)
+        return grad
+
+
+class _LogLoss(torch.autograd.Function):
+    """renderer/losses.cpp:50-142: log-MSE / log-L1 of log-space inputs, with both derivatives."""
+
+    @staticmethod
+    def forward(ctx, log_x, log_y, l1):
+        if log_x.shape != log_y.shape:
+            raise RuntimeError("both args must have the same shape")
+        if log_x.device != log_y.device:
+            raise RuntimeError("both args must be on the same device")
+        if log_x.dtype != log_y.dtype:
+            raise RuntimeError("both args must have the same dtype")
+        log_x, log_y = log_x.contiguous(), log_y.contiguous()
+        out = torch.empty_like(log_x)
+        _n.log_loss(bool(l1), log_x, log_y, out)
+        ctx.save_for_backward(log_x, log_y)
+        ctx.l1 = bool(l1)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        log_x, log_y = ctx.saved_tensors
+        gx, gy = torch.empty_like(log_x), torch.empty_like(log_y)
+        _n.log_loss_backward(ctx.l1, log_x, log_y, grad_output.contiguous(), gx, gy)
+        return gx, gy, None
+
+
+def mul_log(t: torch.Tensor) -> torch.Tensor:
+    """t * log2(t) elementwise, 0 where t < FLT_EPSILON; differentiable (renderer/losses.cpp:45-48)."""
+    return _MulLog.apply(t)
+
+
+def log_mse(log_x: torch.Tensor, log_y: torch.Tensor) -> torch.Tensor:
+    """log((x - y)^2) from log x and log y, elementwise; differentiable (renderer/losses.cpp:95-98)."""
+    return _LogLoss.apply(log_x, log_y, False)
+
+
+def log_l1(log_x: torch.Tensor, log_y: torch.Tensor) -> torch.Tensor:
+    """log|x - y| from log x and log y, elementwise; differentiable (renderer/losses.cpp:139-142)."""
+    return _LogLoss.apply(log_x, log_y, True)
+
+
+def sample_importance(sample_locations: torch.Tensor, density_volume: torch.Tensor, tf: torch.Tensor, tf_mode,
+                      weight_uniform: float, weight_density_gradient: float, weight_opacity: float,
+                      weight_opacity_gradient: float, seed: int) -> torch.Tensor:
+    """Importance sampling of N positions (N x 3 in [0,1]^3) of the volume: a bool tensor (N,)
+    marking the samples taken (renderer/losses.cpp:144-221, bindings/bindings.cpp:661-688).
+    Each sample picks a category with probability proportional to its weight: uniform (always
+    taken), density-gradient norm, opacity, opacity-gradient norm (taken with probability
+    value / maximum over the voxels)."""
+    out = torch.empty(sample_locations.shape[0] if sample_locations.dim() >= 1 else 0, dtype=torch.bool,
+                      device=sample_locations.device)
+    maxima = torch.empty(3, dtype=torch.float32, device=density_volume.device)
+    _n.sample_importance(sample_locations, density_volume, tf, int(tf_mode),
+                         [float(weight_uniform), float(weight_density_gradient), float(weight_opacity),
+                          float(weight_opacity_gradient)], int(seed), maxima, out)
+    return out
+
+
 __all__ = [
     "real3", "real4", "int2", "int3", "VolumeFilterMode", "CameraMode", "TFMode", "BlendMode", "Orientation",
     "CameraPerPixelRays", "CameraReferenceFrame", "mat4", "RendererInputs", "RendererOutputs", "AdjointOutputs",
     "ForwardDifferencesSettings", "GpuTimer", "sync", "use_double_precision", "Renderer", "Camera",
-    "ImplicitEquation", "Volume", "TFPoint", "TFUtils",
+    "ImplicitEquation", "Volume", "TFPoint", "TFUtils", "mul_log", "log_mse", "log_l1", "sample_importance",
 ]

@@ -16,7 +16,9 @@
  *   - launches are asynchronous on the given HIP stream (NULL = default stream).
  *
  * All floating-point tensors are float32 (the reference's real_t with
- * USE_DOUBLE_PRECISION=0); terminationIndex is int32.
+ * USE_DOUBLE_PRECISION=0), except that the elementwise losses also take
+ * float64 (AT_DISPATCH_FLOATING_TYPES in the reference); terminationIndex is
+ * int32.
  */
 #ifndef DVR_H_
 #define DVR_H_
@@ -38,6 +40,8 @@ typedef int32_t dvr_status;
 /* dtypes */
 #define DVR_FLOAT32 0
 #define DVR_INT32 1
+#define DVR_FLOAT64 2 /* the elementwise losses only */
+#define DVR_BOOL 3    /* one byte per element, 0 or 1 (importance-sampling output) */
 
 /* Enum values are those of renderer/renderer_settings.cuh:81-146 and of the
  * pyrenderer enums (bindings/bindings.cpp:215-232). */
@@ -62,7 +66,7 @@ typedef struct dvr_tensor {
     int64_t sizes[6];
     int64_t strides[6]; /* in elements */
     int32_t ndim;
-    int32_t dtype;      /* DVR_FLOAT32 / DVR_INT32 */
+    int32_t dtype;      /* DVR_FLOAT32 / DVR_INT32 / DVR_FLOAT64 / DVR_BOOL */
     int32_t device;     /* HIP device ordinal, -1 = host memory */
 } dvr_tensor;
 
@@ -436,6 +440,54 @@ dvr_status dvr_adam_group_step(dvr_adam_group* group, double lr, double beta1, d
 /* Set-up only (synchronous copies): set or read the step counter. */
 dvr_status dvr_adam_group_set_step(dvr_adam_group* group, int64_t step);
 dvr_status dvr_adam_group_get_step(dvr_adam_group* group, int64_t* step);
+
+/* ---- TF utilities on device tensors ---------------------------------------
+ * Replace renderer::TFUtils::preshadeVolume / findBestFit
+ * (renderer/tf_utils.cpp:173-334, bound at bindings/bindings.cpp:570-633)
+ * and kernel::PreshadeVolume(Adj) / FindBestFit(WithComparison)
+ * (renderer/tf_utils_impl.cu:56-190, :301-520).  Volumes are 1 x X x Y x Z
+ * (densities) or 4 x X x Y x Z (r, g, b, absorption), float32, any strides;
+ * tf is 1 x R x C as for rendering: identity 1 x 2, texture R x 4, linear
+ * R x 5 (R > 1), Gaussian and GaussianLog R x 6.  R*C <= 16384. */
+
+/* color = TF(density) per voxel (tf_mode identity/texture/linear/Gaussian/GaussianLog). */
+dvr_status dvr_preshade_volume(const dvr_tensor* density, const dvr_tensor* tf, int32_t tf_mode, dvr_tensor* color,
+                               void* stream);
+/* adj_density = d<adj_color, TF(density)>/d density per voxel (overwritten, not accumulated). */
+dvr_status dvr_preshade_volume_adjoint(const dvr_tensor* density, const dvr_tensor* tf, int32_t tf_mode,
+                                       const dvr_tensor* adj_color, dvr_tensor* adj_density, void* stream);
+/* density = the best of num_samples random densities per voxel for the target colours
+ * (tf_mode identity/texture/linear/Gaussian).  previous_densities (NULL or not given =
+ * none) adds neighbor_weighting * (d - previous neighbour)^2 for each existing axis
+ * neighbour.  Each voxel draws its own candidates from `seed` and its (x, y, z) index,
+ * so the result does not depend on the launch (the reference's draws do). */
+dvr_status dvr_find_best_fit(const dvr_tensor* color_volume, const dvr_tensor* tf, int32_t tf_mode,
+                             int32_t num_samples, float opacity_weighting, const dvr_tensor* previous_densities,
+                             float neighbor_weighting, uint64_t seed, dvr_tensor* density, void* stream);
+
+/* ---- importance sampling and elementwise losses ---------------------------
+ * Replace renderer::sampleImportance / mul_log / logMSE / logL1
+ * (renderer/losses.cpp:8-221, bound at bindings/bindings.cpp:636-688) and
+ * their kernels (renderer/losses_impl.cu:35-421). */
+
+/* out[i] = whether sample i (sample_locations N x 3 in [0,1]^3, float32) is taken, as
+ * DVR_BOOL (N).  weights: uniform, density gradient, opacity, opacity gradient (>= 0,
+ * normalised here).  maxima: float32 tensor of 3 elements, overwritten with the maxima
+ * of |grad density|, opacity and |grad opacity| over the voxels (computed on the
+ * device; the reference copies them to the host).  Sample i draws its two uniform
+ * numbers from `seed` and i. */
+dvr_status dvr_sample_importance(const dvr_tensor* sample_locations, const dvr_tensor* density, const dvr_tensor* tf,
+                                 int32_t tf_mode, const float weights[4], int32_t seed, dvr_tensor* maxima,
+                                 dvr_tensor* out, void* stream);
+/* Elementwise over contiguous tensors of one shape and dtype (float32 or float64). */
+dvr_status dvr_mul_log(const dvr_tensor* t, dvr_tensor* out, void* stream);
+dvr_status dvr_mul_log_backward(const dvr_tensor* t, const dvr_tensor* grad_out, dvr_tensor* grad_t, void* stream);
+dvr_status dvr_log_mse(const dvr_tensor* log_x, const dvr_tensor* log_y, dvr_tensor* out, void* stream);
+dvr_status dvr_log_mse_backward(const dvr_tensor* log_x, const dvr_tensor* log_y, const dvr_tensor* grad_out,
+                                dvr_tensor* grad_log_x, dvr_tensor* grad_log_y, void* stream);
+dvr_status dvr_log_l1(const dvr_tensor* log_x, const dvr_tensor* log_y, dvr_tensor* out, void* stream);
+dvr_status dvr_log_l1_backward(const dvr_tensor* log_x, const dvr_tensor* log_y, const dvr_tensor* grad_out,
+                               dvr_tensor* grad_log_x, dvr_tensor* grad_log_y, void* stream);
 
 /* ---- GPU timer (bindings/bindings.cpp:99-129) ---------------------------- */
 typedef struct dvr_timer dvr_timer;

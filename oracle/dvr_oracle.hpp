@@ -32,13 +32,14 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace oracle {
 
 enum FilterMode { FilterNearest = 0, FilterTrilinear = 1, FilterPreshaded = 3 };
 enum CameraMode { CameraRayStartDir = 0, CameraInverseViewMatrix = 1, CameraReferenceFrame = 2 };
-enum TFMode { TFIdentity = 0, TFTexture = 1, TFLinear = 2, TFGaussian = 3, TFPreshaded = 5 };
+enum TFMode { TFIdentity = 0, TFTexture = 1, TFLinear = 2, TFGaussian = 3, TFPreshaded = 5, TFGaussianLog = 6 };
 enum BlendMode { BlendBeerLambert = 0, BlendAlpha = 1 };
 enum AdjointMethod { AdjointInversion = 0, AdjointExactReplay = 1 };
 
@@ -310,6 +311,22 @@ inline V4<T> tf_eval(const Inputs<T>& in, int b, T density)
         const T f = (d2 - p0) / (p1 - p0);
         return v4(lerp_(a[0], c[0], f), lerp_(a[1], c[1], f), lerp_(a[2], c[2], f), lerp_(a[3], c[3], f));
     }
+    case TFGaussianLog: {  // renderer_tf.cuh:676-688: log-sum-exp over i of log c_i - (d-mu_i)^2/(2 sigma_i^2)
+        const int R = in.R;
+        T m[4], sum[4] = {T(0), T(0), T(0), T(0)};
+        for (int ch = 0; ch < 4; ++ch) m[ch] = -std::numeric_limits<T>::infinity();
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = 0; i < R; ++i) {
+                const T* r = tf_row(in, b, i);
+                const T ln = -(density - r[4]) * (density - r[4]) / (T(2) * r[5] * r[5]);
+                for (int ch = 0; ch < 4; ++ch) {
+                    const T t = std::log(r[ch]) + ln;
+                    if (pass == 0) m[ch] = std::max(m[ch], t);
+                    else sum[ch] += std::exp(t - m[ch]);
+                }
+            }
+        return v4(m[0] + std::log(sum[0]), m[1] + std::log(sum[1]), m[2] + std::log(sum[2]), m[3] + std::log(sum[3]));
+    }
     default: {  // TFGaussian: sum_i c_i exp(-(d-mu)^2/(2 sigma^2)), column 5 is sigma
         V4<T> out = v4(T(0), T(0), T(0), T(0));
         for (int i = 0; i < in.R; ++i) {
@@ -383,6 +400,18 @@ inline T tf_adjoint(const Inputs<T>& in, int b, T density, V4<T> adj, T* adj_tf 
             atomic_add(&adj_tf[j * C + 4], adj_p1);
         }
         return adj_d2;
+    }
+    case TFGaussianLog: {  // :706-744, the density term (adj_tf is not formed: pre-shading does not ask for it)
+        const V4<T> lse = tf_eval(in, b, density);
+        const T l[4] = {lse.x, lse.y, lse.z, lse.w}, a[4] = {adj.x, adj.y, adj.z, adj.w};
+        T adj_d = T(0);
+        for (int i = 0; i < in.R; ++i) {
+            const T* r = tf_row(in, b, i);
+            const T mu = r[4], s2 = r[5] * r[5];
+            const T ln = -(density - mu) * (density - mu) / (T(2) * s2);
+            for (int ch = 0; ch < 4; ++ch) adj_d += a[ch] * std::exp(std::log(r[ch]) + ln - l[ch]) * (mu - density) / s2;
+        }
+        return adj_d;
     }
     default: {  // TFGaussian :562-614
         T adj_d = T(0);
@@ -1074,6 +1103,163 @@ inline void kink_census(const Inputs<T>& in, const int* term, T eps, T eps_pos, 
         const int y = (int)((i - (long long)b * in.W * in.H) / in.W);
         const int x = (int)(i - (long long)in.W * (y + (long long)in.H * b));
         out[i] = kink_samples(in, x, y, b, term[i], eps, eps_pos, eps_face);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// f4: auxiliary kernels.  Dense volumes are X x Y x Z with z fastest (C order).
+// ---------------------------------------------------------------------------
+template <class T> inline T sq(T v) { return v * v; }
+
+// pre-shading (renderer/tf_utils_impl.cu:17-32): color[ch][v] = TF(density[v])
+template <class T>
+inline void preshade(const Inputs<T>& tf, const T* dens, int64_t n, T* color)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < n; ++v) {
+        const V4<T> c = tf_eval(tf, 0, dens[v]);
+        color[v] = c.x; color[n + v] = c.y; color[2 * n + v] = c.z; color[3 * n + v] = c.w;
+    }
+}
+
+// its adjoint (:96-121): adj_density[v] = d<adj_color[.][v], TF>/d density
+template <class T>
+inline void preshade_adjoint(const Inputs<T>& tf, const T* dens, int64_t n, const T* adj_color, T* adj_dens)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < n; ++v) {
+        const V4<T> a = v4(adj_color[v], adj_color[n + v], adj_color[2 * n + v], adj_color[3 * n + v]);
+        adj_dens[v] = tf_adjoint(tf, 0, dens[v], a, (T*)nullptr);
+    }
+}
+
+// Random streams of the GPU kernels: SplitMix64 (Steele, Lea and Flood 2014)
+// and the 48-bit LCG of java.util.Random (multiplier 0x5DEECE66D, increment 11),
+// whose 24 high bits make a float in [0, 1) (renderer/tf_utils_impl.cu:196-210).
+inline uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+inline uint64_t lcg48_next(uint64_t s) { return (s * 0x5DEECE66Dull + 0xBull) & ((1ull << 48) - 1); }
+inline double lcg48_unit(uint64_t s) { return double(uint32_t(s >> 24)) / 16777216.0; }
+inline uint64_t best_fit_stream(uint64_t seed, uint64_t voxel) { return splitmix64(splitmix64(seed) + voxel) & ((1ull << 48) - 1); }
+
+// cost of density d for one voxel (:239-262, :389-409)
+template <class T>
+inline T best_fit_cost(const Inputs<T>& tf, const T target[4], T d, T opacity_w, const T* nb, const bool* has, T neigh_w)
+{
+    const V4<T> c = tf_eval(tf, 0, d);
+    const T dx = target[0] - c.x, dy = target[1] - c.y, dz = target[2] - c.z;
+    T cost = (dx * dx + dy * dy + dz * dz) + opacity_w * std::log1p(std::fabs(target[3] - c.w));
+    if (nb)
+        for (int j = 0; j < 6; ++j)
+            if (has[j]) cost += neigh_w * (d - nb[j]) * (d - nb[j]);
+    return cost;
+}
+
+// best fit (:225-272, :351-420): color 4 x X x Y x Z, prev (or null) and out X x Y x Z;
+// best_cost (or null) receives the winning cost
+template <class T>
+inline void best_fit(const Inputs<T>& tf, const T* color, int X, int Y, int Z, int num_samples, T opacity_w,
+                     const T* prev, T neigh_w, uint64_t seed, T* out, T* best_cost)
+{
+    const int64_t n = (int64_t)X * Y * Z;
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < n; ++v) {
+        const int x = int(v / ((int64_t)Y * Z)), y = int((v / Z) % Y), z = int(v % Z);
+        const T target[4] = {color[v], color[n + v], color[2 * n + v], color[3 * n + v]};
+        T nb[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        bool has[6] = {x > 0, x < X - 1, y > 0, y < Y - 1, z > 0, z < Z - 1};
+        const int64_t st[6] = {-(int64_t)Y * Z, (int64_t)Y * Z, -Z, Z, -1, 1};
+        if (prev)
+            for (int j = 0; j < 6; ++j)
+                if (has[j]) nb[j] = prev[v + st[j]];
+        uint64_t s = best_fit_stream(seed, (uint64_t)v);
+        T best = T(3.402823466e38), bestD = T(0);
+        for (int k = 0; k < num_samples; ++k) {
+            s = lcg48_next(s);
+            const T d = T(lcg48_unit(s));
+            const T cost = best_fit_cost(tf, target, d, opacity_w, prev ? nb : nullptr, has, neigh_w);
+            if (cost < best) {
+                best = cost;
+                bestD = d;
+            }
+        }
+        out[v] = bestD;
+        if (best_cost) best_cost[v] = best;
+    }
+}
+
+// trilinear fetch of a dense volume at voxel position p (renderer_interpolation.cuh:145-177)
+template <class T> inline T fetch_dense_trilinear(const T* vol, int X, int Y, int Z, T px, T py, T pz)
+{
+    Inputs<T> in{};
+    in.volume = vol; in.vol_b = 1; in.X = X; in.Y = Y; in.Z = Z; in.filter = FilterTrilinear;
+    TriCell<T> c;
+    trilinear_cell(in, 0, v3(px, py, pz), c);
+    return trilinear_value(c);
+}
+
+// importance sampling (renderer/losses_impl.cu:254-292): maxima of |grad density|,
+// opacity and |grad opacity| over the voxels (central differences, clamped at the border)
+template <class T>
+inline void importance_maxima(const Inputs<T>& tf, const T* dens, int X, int Y, int Z, T out[3])
+{
+    out[0] = out[1] = out[2] = T(0);
+    auto at = [&](int x, int y, int z) { return dens[((int64_t)x * Y + y) * Z + z]; };
+    auto op = [&](T d) { return tf_eval(tf, 0, d).w; };
+    for (int x = 0; x < X; ++x)
+        for (int y = 0; y < Y; ++y)
+            for (int z = 0; z < Z; ++z) {
+                const T c = at(x, y, z);
+                const T xm = at(std::max(0, x - 1), y, z), xp = at(std::min(X - 1, x + 1), y, z);
+                const T ym = at(x, std::max(0, y - 1), z), yp = at(x, std::min(Y - 1, y + 1), z);
+                const T zm = at(x, y, std::max(0, z - 1)), zp = at(x, y, std::min(Z - 1, z + 1));
+                const T gd = std::sqrt(sq((xp - xm) / 2) + sq((yp - ym) / 2) + sq((zp - zm) / 2));
+                const T go = std::sqrt(sq((op(xp) - op(xm)) / 2) + sq((op(yp) - op(ym)) / 2) + sq((op(zp) - op(zm)) / 2));
+                out[0] = std::max(out[0], gd);
+                out[1] = std::max(out[1], op(c));
+                out[2] = std::max(out[2], go);
+            }
+}
+
+// the decision for sample i (:298-371) with the GPU's uniform numbers in (0, 1]
+// (two 24-bit fields of splitmix64(splitmix64(seed) + i)); weights are the
+// cumulative normalised thresholds (losses.cpp:188-200, in float as there).
+// margin[i] = |value / max - rndReject| for the rejection branches, +inf for
+// the uniform branch: decisions with a small margin may flip under rounding.
+template <class T>
+inline void sample_importance(const Inputs<T>& tf, const T* dens, int X, int Y, int Z, const T* loc, int64_t N,
+                              const float w[3], const T maxima[3], uint32_t seed, uint8_t* out, T* margin)
+{
+    const uint64_t sm = splitmix64((uint64_t)seed);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        const T px = loc[3 * i] * T(X - 1), py = loc[3 * i + 1] * T(Y - 1), pz = loc[3 * i + 2] * T(Z - 1);
+        const uint64_t h = splitmix64(sm + (uint64_t)i);
+        const float rndSample = float(uint32_t(h >> 40) + 1u) / 16777216.0f;
+        const T rndReject = T(float(uint32_t((h >> 16) & 0xFFFFFFull) + 1u) / 16777216.0f);
+        auto f = [&](T dx, T dy, T dz) { return fetch_dense_trilinear(dens, X, Y, Z, px + dx, py + dy, pz + dz); };
+        auto op = [&](T d) { return tf_eval(tf, 0, d).w; };
+        T value;
+        if (rndSample <= w[0]) {
+            out[i] = 1;
+            if (margin) margin[i] = std::numeric_limits<T>::infinity();
+            continue;
+        } else if (rndSample <= w[1]) {
+            value = std::sqrt(sq((f(1, 0, 0) - f(-1, 0, 0)) / 2) + sq((f(0, 1, 0) - f(0, -1, 0)) / 2) +
+                              sq((f(0, 0, 1) - f(0, 0, -1)) / 2)) / maxima[0];
+        } else if (rndSample <= w[2]) {
+            value = op(f(0, 0, 0)) / maxima[1];
+        } else {
+            value = std::sqrt(sq((op(f(1, 0, 0)) - op(f(-1, 0, 0))) / 2) + sq((op(f(0, 1, 0)) - op(f(0, -1, 0))) / 2) +
+                              sq((op(f(0, 0, 1)) - op(f(0, 0, -1))) / 2)) / maxima[2];
+        }
+        out[i] = value >= rndReject ? 1 : 0;
+        if (margin) margin[i] = std::fabs(value - rndReject);
     }
 }
 

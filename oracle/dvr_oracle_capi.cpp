@@ -183,8 +183,53 @@ template <class T> oracle::Inputs<T> tf_only(int mode, const T* tf, int R, int C
         oracle::generate_rays_adjoint<T>(viewport, B, H, W, fov_y, adj_start, adj_dir, adj_viewport);          \
     }
 
+#define ORACLE_DEFINE_AUX(T, SUF)                                                                                \
+    extern "C" void oracle_preshade_##SUF(int mode, const T* tf, int R, int C, const T* dens, int64_t n, T* color) \
+    {                                                                                                          \
+        oracle::preshade<T>(tf_only<T>(mode, tf, R, C), dens, n, color);                                       \
+    }                                                                                                          \
+    extern "C" void oracle_preshade_adjoint_##SUF(int mode, const T* tf, int R, int C, const T* dens, int64_t n,  \
+                                                  const T* adj_color, T* adj_dens)                              \
+    {                                                                                                          \
+        oracle::preshade_adjoint<T>(tf_only<T>(mode, tf, R, C), dens, n, adj_color, adj_dens);                 \
+    }                                                                                                          \
+    extern "C" void oracle_best_fit_##SUF(int mode, const T* tf, int R, int C, const T* color, int X, int Y,      \
+                                          int Z, int num_samples, T opacity_w, const T* prev, T neigh_w,         \
+                                          uint64_t seed, T* out, T* best_cost)                                   \
+    {                                                                                                          \
+        oracle::best_fit<T>(tf_only<T>(mode, tf, R, C), color, X, Y, Z, num_samples, opacity_w, prev, neigh_w,   \
+                            seed, out, best_cost);                                                             \
+    }                                                                                                          \
+    /* nb: 6 neighbour densities (x-, x+, y-, y+, z-, z+) or null; has: 6 flags */                             \
+    extern "C" T oracle_best_fit_cost_##SUF(int mode, const T* tf, int R, int C, const T* target, T d,          \
+                                            T opacity_w, const T* nb, const int* has, T neigh_w)                \
+    {                                                                                                          \
+        bool h[6] = {false, false, false, false, false, false};                                               \
+        if (has)                                                                                               \
+            for (int j = 0; j < 6; ++j) h[j] = has[j] != 0;                                                    \
+        return oracle::best_fit_cost<T>(tf_only<T>(mode, tf, R, C), target, d, opacity_w, nb, h, neigh_w);     \
+    }                                                                                                          \
+    extern "C" void oracle_importance_maxima_##SUF(int mode, const T* tf, int R, int C, const T* dens, int X,   \
+                                                   int Y, int Z, T* out)                                        \
+    {                                                                                                          \
+        oracle::importance_maxima<T>(tf_only<T>(mode, tf, R, C), dens, X, Y, Z, out);                          \
+    }                                                                                                          \
+    extern "C" void oracle_sample_importance_##SUF(int mode, const T* tf, int R, int C, const T* dens, int X,   \
+                                                   int Y, int Z, const T* loc, int64_t N, const float* w,       \
+                                                   const T* maxima, uint32_t seed, uint8_t* out, T* margin)     \
+    {                                                                                                          \
+        oracle::sample_importance<T>(tf_only<T>(mode, tf, R, C), dens, X, Y, Z, loc, N, w, maxima, seed, out,  \
+                                     margin);                                                                  \
+    }
+
 ORACLE_DEFINE(double, f64)
 ORACLE_DEFINE(float, f32)
+ORACLE_DEFINE_AUX(double, f64)
+ORACLE_DEFINE_AUX(float, f32)
+
+extern "C" uint64_t oracle_splitmix64(uint64_t x) { return oracle::splitmix64(x); }
+extern "C" uint64_t oracle_lcg48_next(uint64_t s) { return oracle::lcg48_next(s); }
+extern "C" uint64_t oracle_best_fit_stream(uint64_t seed, uint64_t voxel) { return oracle::best_fit_stream(seed, voxel); }
 
 extern "C" void oracle_set_threads(int n)
 {

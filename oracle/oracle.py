@@ -69,6 +69,13 @@ def lib():
             getattr(_lib, f"oracle_tf_adjoint_{suf}").restype = ctypes.c_double if suf == "f64" else ctypes.c_float
             getattr(_lib, f"oracle_fetch_{suf}").restype = ctypes.c_double if suf == "f64" else ctypes.c_float
         _lib.oracle_get_threads.restype = ctypes.c_int
+        for suf, ct in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+            getattr(_lib, f"oracle_best_fit_cost_{suf}").restype = ct
+        for name in ("oracle_splitmix64", "oracle_lcg48_next", "oracle_best_fit_stream"):
+            getattr(_lib, name).restype = ctypes.c_uint64
+        _lib.oracle_splitmix64.argtypes = [ctypes.c_uint64]
+        _lib.oracle_lcg48_next.argtypes = [ctypes.c_uint64]
+        _lib.oracle_best_fit_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     return _lib
 
 
@@ -306,3 +313,147 @@ def generate_rays_adjoint(viewport, H, W, fov_y, adj_start, adj_dir, dtype=np.fl
         _ptr(vp), vp.shape[0], H, W, _c(dtype)(fov_y), _ptr(_arr(adj_start, dtype)), _ptr(_arr(adj_dir, dtype)),
         _ptr(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# auxiliary kernels (renderer/tf_utils_impl.cu, renderer/losses_impl.cu).
+# Volumes are dense (X, Y, Z) / (4, X, Y, Z) arrays, tf an (R, C) table.
+# ---------------------------------------------------------------------------
+def preshade(mode, tf, density, dtype=np.float64):
+    """colour (4, X, Y, Z) = TF(density) per voxel (renderer/tf_utils_impl.cu:17-32)."""
+    tf, d = _arr(tf, dtype), _arr(density, dtype)
+    out = np.zeros((4,) + d.shape, dtype=dtype)
+    getattr(lib(), f"oracle_preshade_{_suffix(dtype)}")(int(mode), _ptr(tf), tf.shape[0], tf.shape[1], _ptr(d),
+                                                         ctypes.c_int64(d.size), _ptr(out))
+    return out
+
+
+def preshade_adjoint(mode, tf, density, adj_color, dtype=np.float64):
+    """d<adj_color, TF(density)>/d density per voxel (renderer/tf_utils_impl.cu:96-121)."""
+    tf, d, a = _arr(tf, dtype), _arr(density, dtype), _arr(adj_color, dtype)
+    out = np.zeros(d.shape, dtype=dtype)
+    getattr(lib(), f"oracle_preshade_adjoint_{_suffix(dtype)}")(int(mode), _ptr(tf), tf.shape[0], tf.shape[1],
+                                                                 _ptr(d), ctypes.c_int64(d.size), _ptr(a), _ptr(out))
+    return out
+
+
+def splitmix64(x: int) -> int:
+    return int(lib().oracle_splitmix64(ctypes.c_uint64(x & (2 ** 64 - 1))))
+
+
+def lcg48_next(state: int) -> int:
+    return int(lib().oracle_lcg48_next(ctypes.c_uint64(state)))
+
+
+def best_fit_candidates(seed: int, voxel: int, n: int) -> np.ndarray:
+    """The n candidate densities of voxel index (x*Y + y)*Z + z."""
+    s = int(lib().oracle_best_fit_stream(ctypes.c_uint64(seed), ctypes.c_uint64(voxel)))
+    out = np.empty(n)
+    for k in range(n):
+        s = lcg48_next(s)
+        out[k] = (s >> 24) / 16777216.0
+    return out
+
+
+def best_fit(mode, tf, color, num_samples, opacity_weighting, prev=None, neighbor_weighting=0.0, seed=0,
+             dtype=np.float64):
+    """Best-fit densities (X, Y, Z) and their costs (renderer/tf_utils_impl.cu:225-272, :351-420)."""
+    tf, c = _arr(tf, dtype), _arr(color, dtype)
+    pv = None if prev is None else _arr(prev, dtype)
+    X, Y, Z = c.shape[1:]
+    out, cost = np.zeros((X, Y, Z), dtype=dtype), np.zeros((X, Y, Z), dtype=dtype)
+    ct = _c(dtype)
+    getattr(lib(), f"oracle_best_fit_{_suffix(dtype)}")(
+        int(mode), _ptr(tf), tf.shape[0], tf.shape[1], _ptr(c), X, Y, Z, int(num_samples), ct(opacity_weighting),
+        _ptr(pv), ct(neighbor_weighting), ctypes.c_uint64(seed), _ptr(out), _ptr(cost))
+    return out, cost
+
+
+def best_fit_cost(mode, tf, target, d, opacity_weighting, nb=None, has=None, neighbor_weighting=0.0,
+                  dtype=np.float64):
+    tf, t = _arr(tf, dtype), _arr(target, dtype)
+    nbv = None if nb is None else _arr(nb, dtype)
+    hv = None if has is None else np.ascontiguousarray(np.asarray(has, dtype=np.int32))
+    ct = _c(dtype)
+    return float(getattr(lib(), f"oracle_best_fit_cost_{_suffix(dtype)}")(
+        int(mode), _ptr(tf), tf.shape[0], tf.shape[1], _ptr(t), ct(d), ct(opacity_weighting), _ptr(nbv), _ptr(hv),
+        ct(neighbor_weighting)))
+
+
+def importance_thresholds(weights):
+    """renderer/losses.cpp:188-200 in float32: cumulative weights divided by their sum."""
+    u, dg, o, og = (np.float32(w) for w in weights)
+    dg = np.float32(dg + u)
+    o = np.float32(o + dg)
+    og = np.float32(og + o)
+    return np.array([u / og, dg / og, o / og], dtype=np.float32)
+
+
+def importance_maxima(mode, tf, density, dtype=np.float64):
+    """max |grad density|, max opacity, max |grad opacity| (renderer/losses_impl.cu:254-292)."""
+    tf, d = _arr(tf, dtype), _arr(density, dtype)
+    out = np.zeros(3, dtype=dtype)
+    X, Y, Z = d.shape
+    getattr(lib(), f"oracle_importance_maxima_{_suffix(dtype)}")(int(mode), _ptr(tf), tf.shape[0], tf.shape[1],
+                                                                  _ptr(d), X, Y, Z, _ptr(out))
+    return out
+
+
+def sample_importance(mode, tf, density, locations, weights, maxima, seed, dtype=np.float64):
+    """Decisions (N,) and their margins |value/max - rndReject| (inf for the uniform
+    branch) with the GPU kernel's random numbers (renderer/losses_impl.cu:298-371)."""
+    tf, d, loc = _arr(tf, dtype), _arr(density, dtype), _arr(locations, dtype)
+    mx = _arr(maxima, dtype)
+    w = np.ascontiguousarray(importance_thresholds(weights))
+    N = loc.shape[0]
+    out = np.zeros(N, dtype=np.uint8)
+    margin = np.zeros(N, dtype=dtype)
+    X, Y, Z = d.shape
+    getattr(lib(), f"oracle_sample_importance_{_suffix(dtype)}")(
+        int(mode), _ptr(tf), tf.shape[0], tf.shape[1], _ptr(d), X, Y, Z, _ptr(loc), ctypes.c_int64(N), _ptr(w),
+        _ptr(mx), ctypes.c_uint32(seed & 0xFFFFFFFF), _ptr(out), _ptr(margin))
+    return out.astype(bool), margin
+
+
+# elementwise losses (renderer/losses_impl.cu:35-240 with renderer/renderer_utils.cuh:180-298), FP64
+_FLT_EPSILON = 1.192092896e-07
+
+
+def _log1mexp(u):
+    """log(1 - e^u) for u <= 0 without cancellation (Maechler 2012)."""
+    u = np.asarray(u, dtype=np.float64)
+    with np.errstate(divide="ignore"):
+        return np.where(u > -np.log(2.0), np.log(-np.expm1(np.minimum(u, 0.0))),
+                        np.log1p(-np.exp(np.minimum(u, 0.0))))
+
+
+def mul_log(t):
+    t = np.asarray(t, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(t < _FLT_EPSILON, 0.0, t * np.log2(np.maximum(t, _FLT_EPSILON)))
+
+
+def mul_log_backward(t, grad):
+    t = np.asarray(t, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(t < _FLT_EPSILON, 0.0, grad * (np.log2(np.maximum(t, _FLT_EPSILON)) + 1.0 / np.log(2.0)))
+
+
+def log_mse(log_x, log_y):
+    a, b = np.maximum(log_x, log_y), np.minimum(log_x, log_y)
+    return 2.0 * (_log1mexp(b - a) + a)
+
+
+def log_l1(log_x, log_y):
+    a, b = np.maximum(log_x, log_y), np.minimum(log_x, log_y)
+    return _log1mexp(b - a) + a
+
+
+def log_mse_backward(log_x, log_y, grad):
+    with np.errstate(divide="ignore"):
+        return 2.0 * grad / -np.expm1(log_y - log_x), 2.0 * grad / -np.expm1(log_x - log_y)
+
+
+def log_l1_backward(log_x, log_y, grad):
+    with np.errstate(divide="ignore"):
+        return grad / -np.expm1(log_y - log_x), grad / -np.expm1(log_x - log_y)

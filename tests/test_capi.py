@@ -294,3 +294,70 @@ def test_compare_to_image_validation():
     host = tensor((1, 6, 8, 4), -1)
     assert l.dvr_compare_to_image(ctypes.byref(host), ctypes.byref(g), ctypes.byref(ref), ctypes.byref(d4),
                                   ctypes.byref(g4), 0, None) == 3
+
+
+# ---------------------------------------------------------------------------
+# auxiliary kernels: validation before any HIP call (renderer/tf_utils.cpp:173-334,
+# renderer/losses.cpp:8-221)
+# ---------------------------------------------------------------------------
+def test_preshade_validation_messages():
+    l = lib()
+    d, c = tensor((1, 4, 4, 4), 0), tensor((4, 4, 4, 4), 0)
+    st = l.dvr_preshade_volume(ctypes.byref(d), ctypes.byref(tensor((1, 2, 6), 0)), 5, ctypes.byref(c), None)
+    assert st == 1 and b"unknown tf enum value: TFPreshaded" in l.dvr_last_error()
+    st = l.dvr_preshade_volume(ctypes.byref(d), ctypes.byref(tensor((1, 3, 4), 0)), 0, ctypes.byref(c), None)
+    assert st == 1 and b"tf must have 1 entries at dimension 1" in l.dvr_last_error()
+    st = l.dvr_preshade_volume(ctypes.byref(d), ctypes.byref(tensor((1, 128, 256), 0)), 1, ctypes.byref(c), None)
+    assert st == 1 and b"tf must have 4 entries at dimension 2" in l.dvr_last_error()
+    big = tensor((1, 8192, 6), 0)
+    st = l.dvr_preshade_volume(ctypes.byref(d), ctypes.byref(big), 6, ctypes.byref(c), None)
+    assert st == 3 and b"at most 16384 entries" in l.dvr_last_error()
+    st = l.dvr_preshade_volume(ctypes.byref(tensor((1, 4, 4, 4), -1)), ctypes.byref(tensor((1, 2, 6), -1)), 3,
+                               ctypes.byref(tensor((4, 4, 4, 4), -1)), None)
+    assert st == 3 and b"GPU only" in l.dvr_last_error()
+    st = l.dvr_preshade_volume(ctypes.byref(d), ctypes.byref(tensor((1, 2, 6), 0)), 3,
+                               ctypes.byref(tensor((4, 4, 4, 5), 0)), None)
+    assert st == 1 and b"color must have 4 entries at dimension 3" in l.dvr_last_error()
+
+
+def test_best_fit_and_importance_validation_messages():
+    l = lib()
+    c, tf, out = tensor((4, 5, 5, 5), 0), tensor((1, 8, 4), 0), tensor((1, 5, 5, 5), 0)
+    seed = ctypes.c_uint64(1)
+    st = l.dvr_find_best_fit(ctypes.byref(c), ctypes.byref(tf), 1, 0, ctypes.c_float(1.0), None,
+                             ctypes.c_float(0.0), seed, ctypes.byref(out), None)
+    assert st == 1 and b"numSamples must be >= 1, but is 0" in l.dvr_last_error()
+    st = l.dvr_find_best_fit(ctypes.byref(c), ctypes.byref(tf), 1, 8, ctypes.c_float(-2.0), None,
+                             ctypes.c_float(0.0), seed, ctypes.byref(out), None)
+    assert st == 1 and b"opacity weighting must be >= 0" in l.dvr_last_error()
+    st = l.dvr_find_best_fit(ctypes.byref(tensor((3, 5, 5, 5), 0)), ctypes.byref(tf), 1, 8, ctypes.c_float(1.0),
+                             None, ctypes.c_float(0.0), seed, ctypes.byref(out), None)
+    assert st == 1 and b"colorVolume must have 4 entries at dimension 0" in l.dvr_last_error()
+    w = (ctypes.c_float * 4)(1, 0, 0, 0)
+    mx, flags = tensor((3,), 0), tensor((10,), 0, dtype=3)
+    st = l.dvr_sample_importance(ctypes.byref(tensor((10, 3), -1)), ctypes.byref(tensor((1, 5, 5, 5), -1)),
+                                 ctypes.byref(tensor((1, 8, 4), -1)), 1, w, 0, ctypes.byref(tensor((3,), -1)),
+                                 ctypes.byref(tensor((10,), -1, dtype=3)), None)
+    assert st == 3 and b"sampleImportance is only supported on CUDA" in l.dvr_last_error()
+    st = l.dvr_sample_importance(ctypes.byref(tensor((10, 3), 0)), ctypes.byref(tensor((1, 5, 5, 5), 0)),
+                                 ctypes.byref(tf), 6, w, 0, ctypes.byref(mx), ctypes.byref(flags), None)
+    assert st == 1 and l.dvr_last_error() == b"unknown tf enum value"
+    wz = (ctypes.c_float * 4)(0, 0, 0, 0)
+    st = l.dvr_sample_importance(ctypes.byref(tensor((10, 3), 0)), ctypes.byref(tensor((1, 5, 5, 5), 0)),
+                                 ctypes.byref(tf), 1, wz, 0, ctypes.byref(mx), ctypes.byref(flags), None)
+    assert st == 1 and b"at least one weight parameter must be positive" in l.dvr_last_error()
+
+
+def test_elementwise_loss_validation_messages():
+    l = lib()
+    a, b = tensor((10, 3), 0), tensor((10, 4), 0)
+    st = l.dvr_log_mse(ctypes.byref(a), ctypes.byref(b), ctypes.byref(tensor((10, 3), 0)), None)
+    assert st == 1 and b"same shape" in l.dvr_last_error()
+    st = l.dvr_log_l1(ctypes.byref(a), ctypes.byref(tensor((10, 3), 0, dtype=2)), ctypes.byref(a), None)
+    assert st == 1 and b"same dtype" in l.dvr_last_error()
+    nc = tensor((10, 3), 0)
+    nc.strides[0], nc.strides[1] = 1, 10
+    st = l.dvr_mul_log(ctypes.byref(nc), ctypes.byref(tensor((10, 3), 0)), None)
+    assert st == 1 and b"t must be contiguous" in l.dvr_last_error()
+    st = l.dvr_mul_log(ctypes.byref(tensor((10,), 0, dtype=1)), ctypes.byref(tensor((10,), 0, dtype=1)), None)
+    assert st == 1 and b"torch.float32 or torch.float64" in l.dvr_last_error()
