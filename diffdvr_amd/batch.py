@@ -109,12 +109,14 @@ class BatchPlan:
     loss_out : optional 1-element float32 tensor; the loss value is added to it (cleared by zeroing runs)
     forward_only : render colour and termination only
     checkpoint_interval, wave_width, grid : kernel options (see :meth:`autotune`)
+    density_store : the forward march stores every sample's density for the reverse march (default);
+        False re-interpolates them instead (no store: grid x samples-per-ray x 64 floats less memory)
     """
 
     def __init__(self, volumes: Sequence[BatchVolume], tfs: Sequence[BatchTf], views: Sequence[BatchView], *,
                  volume_filter_mode=1, tf_mode=1, blend_mode=0, blending_early_out: float = 1.0 - 1e-5,
                  loss: Optional[str] = None, loss_out: Optional[torch.Tensor] = None, forward_only: bool = False,
-                 checkpoint_interval: int = 16, wave_width: int = 8, grid: int = 0):
+                 checkpoint_interval: int = 16, wave_width: int = 8, grid: int = 0, density_store: bool = True):
         if not views:
             raise ValueError("a batch needs at least one view")
         self.volumes, self.tfs, self.views = list(volumes), list(tfs), list(views)
@@ -166,7 +168,7 @@ class BatchPlan:
         self._h = _n._BatchPlan(vt, tt, wt, _mode(volume_filter_mode), tfm, _mode(blend_mode),
                                 float(blending_early_out), loss_kind, _ptr(loss_out), bool(forward_only),
                                 dev.index if dev.index is not None else torch.cuda.current_device(),
-                                int(checkpoint_interval), int(wave_width), int(grid))
+                                int(checkpoint_interval), int(wave_width), int(grid), not density_store)
 
     def _stream(self) -> int:
         return int(torch.cuda.current_stream(self.device).cuda_stream)
@@ -180,18 +182,19 @@ class BatchPlan:
 
     @property
     def options(self) -> dict:
-        k, w, g = self._h.options()
-        return {"checkpoint_interval": k, "wave_width": w, "grid": g}
+        k, w, g, ri = self._h.options()
+        return {"checkpoint_interval": k, "wave_width": w, "grid": g, "density_store": not ri}
 
-    def set_options(self, checkpoint_interval: int = 16, wave_width: int = 8, grid: int = 0) -> None:
-        self._h.set_options(int(checkpoint_interval), int(wave_width), int(grid))
+    def set_options(self, checkpoint_interval: int = 16, wave_width: int = 8, grid: int = 0,
+                    density_store: bool = True) -> None:
+        self._h.set_options(int(checkpoint_interval), int(wave_width), int(grid), not density_store)
 
     def autotune(self, iterations: int = 3) -> dict:
         """Time every (K, wave width) candidate on this batch, keep the fastest.
 
         Outputs are overwritten and the gradient buffers left zeroed."""
-        k, w, g = self._h.autotune(int(iterations), self._stream())
-        return {"checkpoint_interval": k, "wave_width": w, "grid": g}
+        k, w, g, ri = self._h.autotune(int(iterations), self._stream())
+        return {"checkpoint_interval": k, "wave_width": w, "grid": g, "density_store": not ri}
 
     @property
     def workspace_bytes(self) -> int:

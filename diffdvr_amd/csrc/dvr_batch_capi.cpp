@@ -27,6 +27,8 @@ using dvr_host::guarded;
 
 namespace {
 
+constexpr double kMaxDensityStore = 2.0 * 1024 * 1024 * 1024;  // bytes
+
 template <class T> T* dev_alloc(size_t n)
 {
     void* p = nullptr;
@@ -77,6 +79,7 @@ struct dvr_batch_plan {
     double maxSamples = 0;  // upper bound of samples per ray over the views
     // layout
     int K = 16, waveWidth = 8, gridReq = 0, grid = 0, nTiles = 0, cap = 1;
+    bool reinterpolate = false;  // no density store
     int cus = 0, resident = 0;  // compute units, resident workgroups of the current instance
     int qBegin[dvr::kNumXcd + 1] = {};
     size_t lds = 0;
@@ -90,11 +93,15 @@ struct dvr_batch_plan {
     long long maxSegAll = 0;
     float* dCk = nullptr;
     size_t ckFloats = 0;
+    float* dDs = nullptr;   // density store of the forward march (adjoint plans), or NULL
+    size_t dsFloats = 0;
+    int dsCap = 0;
     std::string signature;  // autotune cache key
 
     ~dvr_batch_plan()
     {
         dev_free(dViews); dev_free(dVols); dev_free(dTfs); dev_free(dQueues); dev_free(dSegsAll); dev_free(dCk);
+        dev_free(dDs);
     }
 
     // floats between the TF-gradient table copies (odd: the copies start in different LDS banks);
@@ -113,6 +120,7 @@ struct dvr_batch_plan {
     {
         dvr::BatchParams P{};
         P.views = dViews; P.vols = dVols; P.tfs = dTfs; P.queues = dQueues; P.ck = dCk; P.lossOut = lossOut;
+        P.ds = dsCap > 0 ? dDs : nullptr; P.dsCap = dsCap;
         P.nViews = (int)views.size(); P.nTiles = nTiles; P.ckCap = cap;
         P.ckEvery = K > 32 ? K / 32 : 1;
         for (int i = 0; i <= dvr::kNumXcd; ++i) P.qBegin[i] = qBegin[i];
@@ -166,6 +174,21 @@ struct dvr_batch_plan {
             dCk = dev_alloc<float>(need);
             ckFloats = need;
         }
+        // Density store: the forward march keeps every sample's density, so the reverse march
+        // reads each segment back (one 4-byte load per sample, all in flight) instead of
+        // re-interpolating it (eight gathers per new cell).  Sized for the longest possible ray;
+        // plans whose store would exceed kMaxDensityStore bytes re-interpolate.
+        const double dsWant = std::ceil(maxSamples) + 2.0;
+        const double dsBytes = double(grid) * dsWant * dvr::kBatchBlock * sizeof(float);
+        dsCap = (gm && !reinterpolate && dsBytes <= double(kMaxDensityStore)) ? (int)dsWant : 0;
+        const size_t dsNeed = size_t(grid) * size_t(dsCap) * dvr::kBatchBlock;
+        if (dsNeed > dsFloats) {
+            dev_free(dDs);
+            dDs = nullptr;
+            dsFloats = 0;
+            dDs = dev_alloc<float>(dsNeed);
+            dsFloats = dsNeed;
+        }
         check_hip(hipMemcpy(dViews, views.data(), views.size() * sizeof(dvr::BView), hipMemcpyHostToDevice),
                   "hipMemcpy views");
     }
@@ -211,6 +234,7 @@ void set_options(dvr_batch_plan* p, const dvr_batch_options* o)
     p->K = K;
     p->waveWidth = ww;
     p->gridReq = o ? o->grid : 0;
+    p->reinterpolate = o && o->reinterpolate != 0;
     p->layout();
 }
 
@@ -438,6 +462,7 @@ dvr_status dvr_batch_plan_get_options(const dvr_batch_plan* plan, dvr_batch_opti
         options->checkpoint_interval = plan->K;
         options->wave_width = plan->waveWidth;
         options->grid = plan->grid;
+        options->reinterpolate = plan->reinterpolate ? 1 : 0;
     });
 }
 
@@ -458,10 +483,11 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
         DeviceGuard g(plan->device);
         hipStream_t s = (hipStream_t)stream;
         const int gridReq = plan->gridReq;
+        const std::string tuneKey = plan->signature + (plan->reinterpolate ? "|reinterpolate" : "");
         Tuned best{0, 0, 0};
         {
             std::lock_guard<std::mutex> lk(g_tuneMu);
-            auto it = g_tuneCache.find(plan->signature);
+            auto it = g_tuneCache.find(tuneKey);
             if (it != g_tuneCache.end()) best = it->second;
         }
         if (best.K == 0) {
@@ -469,9 +495,10 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
             hipEvent_t e0, e1;
             check_hip(hipEventCreate(&e0), "hipEventCreate");
             check_hip(hipEventCreate(&e1), "hipEventCreate");
+            const int reint = plan->reinterpolate ? 1 : 0;
             auto time_it = [&](int K, int ww, int g) {
                 check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-                dvr_batch_options o{K, ww, g};
+                dvr_batch_options o{K, ww, g, reint};
                 set_options(plan, &o);
                 plan->run(0, s);  // warm-up
                 check_hip(hipEventRecord(e0, s), "hipEventRecord");
@@ -487,7 +514,7 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
                 Tuned cur{16, 8, 0};
                 float bestMs = 1e30f;
                 {
-                    dvr_batch_options o{cur.K, cur.ww, 0};
+                    dvr_batch_options o{cur.K, cur.ww, 0, reint};
                     set_options(plan, &o);
                 }
                 const int cus = plan->cus, resident = plan->resident;
@@ -519,16 +546,17 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             std::lock_guard<std::mutex> lk(g_tuneMu);
-            g_tuneCache[plan->signature] = best;
+            g_tuneCache[tuneKey] = best;
         }
         check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-        dvr_batch_options o{best.K, best.ww, best.grid};
+        dvr_batch_options o{best.K, best.ww, best.grid, plan->reinterpolate ? 1 : 0};
         set_options(plan, &o);
         plan->zero(true, s);
         if (chosen) {
             chosen->checkpoint_interval = plan->K;
             chosen->wave_width = plan->waveWidth;
             chosen->grid = plan->grid;
+            chosen->reinterpolate = plan->reinterpolate ? 1 : 0;
         }
     });
 }
@@ -536,7 +564,7 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
 int64_t dvr_batch_plan_workspace_bytes(const dvr_batch_plan* plan)
 {
     if (!plan) return -1;
-    return (int64_t)(plan->ckFloats * sizeof(float) + plan->views.size() * sizeof(dvr::BView) +
+    return (int64_t)((plan->ckFloats + plan->dsFloats) * sizeof(float) + plan->views.size() * sizeof(dvr::BView) +
                      plan->vols.size() * sizeof(dvr::BVol) + plan->tfs.size() * sizeof(dvr::BTf) +
                      plan->nSegsAll * sizeof(dvr::ZeroSeg) + dvr::kNumXcd * sizeof(unsigned));
 }

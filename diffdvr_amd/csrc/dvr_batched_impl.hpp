@@ -73,6 +73,8 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
     const int lx = lane & (ww - 1);
     const int ly = lane >> B.waveShift;
     float* ck = ADJ ? B.ck + size_t(blockIdx.x) * size_t(B.ckCap) * kBatchBlock + threadIdx.x : nullptr;
+    // this lane's density store: sample i at ds[i * kBatchBlock]
+    float* ds = ADJ && B.ds ? B.ds + size_t(blockIdx.x) * size_t(B.dsCap) * kBatchBlock + threadIdx.x : nullptr;
     const bool priv = B.usePriv != 0;
     int qoff = 0;
     int curTf = -1, curRC = 0;
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
             Acc acc;
             int n;
             march_forward<FILTER, TFM, BLEND>(VV, T, r, B.earlyOut, -1, acc, n, ck, kBatchBlock, K * B.ckEvery,
-                                              ADJ ? B.ckCap : 0);
+                                              ADJ ? B.ckCap : 0, ds, kBatchBlock, B.dsCap);
             const unsigned pix = unsigned(y * V.W + x);
             gst(V.color, 4 * pix, acc.r);
             gst(V.color, 4 * pix + 1, acc.g);
@@ -171,10 +173,14 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
                 f3 adj_start = mk3(0.f, 0.f, 0.f), adj_dir = mk3(0.f, 0.f, 0.f);
                 float adj_step = 0.0f;
                 const bool wantVol = B.wantVol && vol.adj != nullptr;
+                // the store holds every sample of every active ray of the wave (else re-interpolate;
+                // the choice is uniform over the wave, so the segment loop does not diverge)
+                const bool stored = ds != nullptr && __all(n <= B.dsCap);
                 march_reverse<FILTER, TFM, BLEND, K, GM>(VV, T, r, n, adj_acc, ck, kBatchBlock, B.ckCap, B.ckEvery, sink,
                                                          curWantTf,
                                                          wantVol, wantCam, vol.adj, vol.Y * vol.Z, vol.Z, 1,
-                                                         adj_start, adj_dir, adj_step);
+                                                         adj_start, adj_dir, adj_step, stored ? ds : nullptr,
+                                                         kBatchBlock);
                 if constexpr ((GM & 4) != 0) {
                     if (wantCam) {
                         // ray (eye, normalize(front + fx tanX right + fy tanY up)) -> viewport rows

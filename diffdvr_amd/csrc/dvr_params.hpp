@@ -119,6 +119,8 @@ struct BatchParams {
     const BTf* tfs;
     unsigned* queues;           // kNumXcd tile counters, zeroed before every launch
     float* ck;                  // checkpoints [workgroup][cap][kBatchBlock]
+    float* ds;                  // sample densities [workgroup][dsCap][kBatchBlock] of the forward march, or NULL
+    int dsCap;                  // samples per ray the density store holds
     float* lossOut;             // scalar loss accumulator or NULL
     int nViews, nTiles, ckCap;
     int ckEvery;                // checkpoint interval = K * ckEvery samples

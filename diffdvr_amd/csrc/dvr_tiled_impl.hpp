@@ -56,7 +56,8 @@ __device__ __forceinline__ void stage_tf(const RenderParams& P, int b, float* s_
 // ---------------------------------------------------------------------------
 template <int FILTER, int TFM, int BLEND>
 __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const Ray& r, float earlyOut, int frozen,
-                                              Acc& acc, int& n, float* ck, int ckS, int K, int cap)
+                                              Acc& acc, int& n, float* ck, int ckS, int K, int cap,
+                                              float* dsw = nullptr, int dsS = 0, int dsCap = 0)
 {
     acc = Acc{0.f, 0.f, 0.f, 0.f, 1.f};
     const float tEarly = 1.0f - earlyOut;
@@ -78,6 +79,7 @@ __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const R
             started = true;
         }
         const float d = pipe.value(p);
+        if (dsw && i < dsCap) *(gfloat*)(dsw + i * dsS) = d;
         const float tn = r.t0 + float(i + 1) * r.step;
         const f3 pn = sample_pos(r, tn);
         pipe.request(V, pn);
@@ -281,7 +283,8 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                                               const float* ck, int ckS, int cap, int ckEvery, Sink& sink,
                                               bool wantTf,
                                               bool wantVol, bool wantPos, float* avol, int avx, int avy, int avz,
-                                              f3& adj_start, f3& adj_dir, float& adj_step)
+                                              f3& adj_start, f3& adj_dir, float& adj_step,
+                                              const float* dsr = nullptr, int dsS = 0)
 {
     constexpr bool CT_TF = (GM & 1) != 0, CT_VOL = (GM & 2) != 0, CT_POS = (GM & 4) != 0;
     wantTf = CT_TF && wantTf;
@@ -315,15 +318,30 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
             cell_invalidate(cc);
             for (int i = c * K * ckEvery; i < i0; ++i) {
                 const float t = r.t0 + float(i) * r.step;
-                const float d = fetch<FILTER>(V, sample_pos(r, t), cc);
+                const float d = dsr ? *(const gfloat*)(dsr + i * dsS) : fetch<FILTER>(V, sample_pos(r, t), cc);
                 float alpha, keep;
                 alpha_keep<BLEND>(tf_absorption<TFM>(T, d), r.step, alpha, keep);
                 Tr *= keep;
             }
         }
-        // recompute the segment forward (fetches pipelined one sample ahead),
-        // keep density and transmittance-before per sample
-        {
+        // recompute the segment forward: densities from the forward march's store
+        // (all loads of the segment in flight at once), or re-interpolated with the
+        // fetches pipelined one sample ahead; keep density and transmittance-before
+        if (dsr) {
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (j < cnt) dv[j] = *(const gfloat*)(dsr + (i0 + j) * dsS);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (j < cnt) {
+                    tin[j] = Tr;
+                    float alpha, keep;
+                    alpha_keep<BLEND>(tf_absorption<TFM>(T, dv[j]), r.step, alpha, keep);
+                    if constexpr (kKeepAK) { al[j] = alpha; kp[j] = keep; }
+                    Tr *= keep;
+                }
+            }
+        } else {
             FetchPipe<FILTER> pipe;
             f3 p = sample_pos(r, r.t0 + float(i0) * r.step);
             pipe.start(V, p);

@@ -304,7 +304,7 @@ template <class T> T* ptr_of(const py::handle& h) { return reinterpret_cast<T*>(
 
 std::unique_ptr<BatchPlanHandle> make_plan(py::list volumes, py::list tfs, py::list views, int filter, int tfm,
                                            int blend, float early, int loss, uintptr_t loss_out, bool forward_only,
-                                           int device, int K, int ww, int grid)
+                                           int device, int K, int ww, int grid, bool reinterpolate)
 {
     std::vector<dvr_batch_volume> vs;
     for (auto h : volumes) {
@@ -359,7 +359,7 @@ std::unique_ptr<BatchPlanHandle> make_plan(py::list volumes, py::list tfs, py::l
     b.volume_filter_mode = filter; b.tf_mode = tfm; b.blend_mode = blend; b.blending_early_out = early;
     b.loss = loss; b.loss_out = reinterpret_cast<float*>(loss_out); b.forward_only = forward_only ? 1 : 0;
     b.device = device;
-    dvr_batch_options o{K, ww, grid};
+    dvr_batch_options o{K, ww, grid, reinterpolate ? 1 : 0};
     std::unique_ptr<BatchPlanHandle> hdl(new BatchPlanHandle());
     check(dvr_batch_plan_create(&b, &o, &hdl->plan));
     return hdl;
@@ -369,7 +369,7 @@ py::tuple plan_options(const BatchPlanHandle& h)
 {
     dvr_batch_options o;
     check(dvr_batch_plan_get_options(h.plan, &o));
-    return py::make_tuple(o.checkpoint_interval, o.wave_width, o.grid);
+    return py::make_tuple(o.checkpoint_interval, o.wave_width, o.grid, o.reinterpolate != 0);
 }
 
 struct AdamHandle {
@@ -741,7 +741,7 @@ PYBIND11_MODULE(_pyrenderer, m)
         .def(py::init(&make_plan), py::arg("volumes"), py::arg("tfs"), py::arg("views"), py::arg("filter"),
              py::arg("tf_mode"), py::arg("blend"), py::arg("early_out"), py::arg("loss"), py::arg("loss_out"),
              py::arg("forward_only"), py::arg("device"), py::arg("checkpoint_interval"), py::arg("wave_width"),
-             py::arg("grid"))
+             py::arg("grid"), py::arg("reinterpolate") = false)
         .def("run", [](BatchPlanHandle& h, int flags, uintptr_t stream) {
             check(dvr_batch_plan_run(h.plan, flags, reinterpret_cast<void*>(stream)));
         })
@@ -749,14 +749,14 @@ PYBIND11_MODULE(_pyrenderer, m)
             check(dvr_batch_plan_zero_gradients(h.plan, reinterpret_cast<void*>(stream)));
         })
         .def("options", &plan_options)
-        .def("set_options", [](BatchPlanHandle& h, int K, int ww, int grid) {
-            dvr_batch_options o{K, ww, grid};
+        .def("set_options", [](BatchPlanHandle& h, int K, int ww, int grid, bool reinterpolate) {
+            dvr_batch_options o{K, ww, grid, reinterpolate ? 1 : 0};
             check(dvr_batch_plan_set_options(h.plan, &o));
-        })
+        }, py::arg("checkpoint_interval"), py::arg("wave_width"), py::arg("grid"), py::arg("reinterpolate") = false)
         .def("autotune", [](BatchPlanHandle& h, int iterations, uintptr_t stream) {
             dvr_batch_options o;
             check(dvr_batch_plan_autotune(h.plan, iterations, &o, reinterpret_cast<void*>(stream)));
-            return py::make_tuple(o.checkpoint_interval, o.wave_width, o.grid);
+            return py::make_tuple(o.checkpoint_interval, o.wave_width, o.grid, o.reinterpolate != 0);
         })
         .def("workspace_bytes", [](const BatchPlanHandle& h) { return dvr_batch_plan_workspace_bytes(h.plan); });
 

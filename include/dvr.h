@@ -384,6 +384,9 @@ typedef struct dvr_batch_options {
     int32_t checkpoint_interval; /* K in {8,16,32,64}; 0 = 16 */
     int32_t wave_width;          /* rays along x of a wave's tile: 4, 8, 16 or 32 (tile 4x16 .. 32x2); 0 = 8 */
     int32_t grid;                /* persistent workgroups; 0 = 2 per compute unit (within residency) */
+    int32_t reinterpolate;       /* 0: the forward march stores every sample's density and the reverse march
+                                    reads it back (when the store fits in 2 GiB); 1: the reverse march
+                                    re-interpolates (no store) */
 } dvr_batch_options;
 
 typedef struct dvr_batch_plan dvr_batch_plan;

@@ -292,12 +292,33 @@ def test_every_option_gives_the_same_gradients():
     for K in (8, 16, 32, 64):
         for ww in (4, 8, 16, 32):
             for grid in (0, 8, 37):
-                plan.set_options(K, ww, grid)
+                plan.set_options(K, ww, grid, density_store=(grid != 8))
                 plan.run()
                 torch.cuda.synchronize()
                 for a, b in zip(ref, case.g_tfs + case.g_vols + case.g_cams):
                     e = norm_err(a.double().cpu().numpy(), b.double().cpu().numpy())
                     assert e <= 1e-5, (K, ww, grid, e)
+
+
+@pytest.mark.parametrize("filter,tf_mode,blend", [(1, 1, 0), (1, 3, 1), (0, 2, 0), (1, 0, 1)])
+def test_density_store_matches_reinterpolation(filter, tf_mode, blend):
+    """The reverse march reads the forward march's stored densities (default) or re-interpolates them:
+    the same densities, so colours agree bitwise and gradients up to the order of atomic additions."""
+    case = Case(7700 + 10 * filter + tf_mode, filter, tf_mode, blend)
+    results = []
+    for store in (True, False):
+        plan = case.plan(checkpoint_interval=16, wave_width=8, grid=0, density_store=store)
+        assert plan.options["density_store"] == store
+        plan.run()
+        torch.cuda.synchronize()
+        results.append(([v.color.clone() for v in case.views],
+                        [g.clone() for g in case.g_tfs + case.g_vols + case.g_cams if g is not None]))
+    (c0, g0), (c1, g1) = results
+    for a, b in zip(c0, c1):
+        assert torch.equal(a, b)
+    for a, b in zip(g0, g1):
+        e = norm_err(a.double().cpu().numpy(), b.double().cpu().numpy())
+        assert e <= 1e-6, e
 
 
 def test_adam_matches_torch():
