@@ -79,7 +79,7 @@ __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const R
             started = true;
         }
         const float d = pipe.value(p);
-        if (dsw && i < dsCap) *(gfloat*)(dsw + i * dsS) = d;
+        if (dsw) *(gfloat*)(dsw + min(i, dsCap - 1) * dsS) = d;  // past the capacity: never read back
         const float tn = r.t0 + float(i + 1) * r.step;
         const f3 pn = sample_pos(r, tn);
         pipe.request(V, pn);
@@ -328,18 +328,18 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
         // (all loads of the segment in flight at once), or re-interpolated with the
         // fetches pipelined one sample ahead; keep density and transmittance-before
         if (dsr) {
+            // no per-sample predicate: indices past the ray's last sample re-read that sample
+            // (in bounds), and the values formed for them are never used (the walk skips
+            // j >= cnt; Tr restarts from a checkpoint every segment)
 #pragma unroll
-            for (int j = 0; j < K; ++j)
-                if (j < cnt) dv[j] = *(const gfloat*)(dsr + (i0 + j) * dsS);
+            for (int j = 0; j < K; ++j) dv[j] = *(const gfloat*)(dsr + min(i0 + j, n - 1) * dsS);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                if (j < cnt) {
-                    tin[j] = Tr;
-                    float alpha, keep;
-                    alpha_keep<BLEND>(tf_absorption<TFM>(T, dv[j]), r.step, alpha, keep);
-                    if constexpr (kKeepAK) { al[j] = alpha; kp[j] = keep; }
-                    Tr *= keep;
-                }
+                tin[j] = Tr;
+                float alpha, keep;
+                alpha_keep<BLEND>(tf_absorption<TFM>(T, dv[j]), r.step, alpha, keep);
+                if constexpr (kKeepAK) { al[j] = alpha; kp[j] = keep; }
+                Tr *= keep;
             }
         } else {
             FetchPipe<FILTER> pipe;
