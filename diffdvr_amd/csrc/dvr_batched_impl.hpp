@@ -176,7 +176,8 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
                 // the store holds every sample of every active ray of the wave (else re-interpolate;
                 // the choice is uniform over the wave, so the segment loop does not diverge)
                 const bool stored = ds != nullptr && __all(n <= B.dsCap);
-                march_reverse<FILTER, TFM, BLEND, K, GM>(VV, T, r, n, adj_acc, ck, kBatchBlock, B.ckCap, B.ckEvery, sink,
+                march_reverse<FILTER, TFM, BLEND, K, GM, SinkTiled<TFM>, true>(VV, T, r, n, adj_acc, ck, kBatchBlock,
+                                                                               B.ckCap, B.ckEvery, sink,
                                                          curWantTf,
                                                          wantVol, wantCam, vol.adj, vol.Y * vol.Z, vol.Z, 1,
                                                          adj_start, adj_dir, adj_step, stored ? ds : nullptr,

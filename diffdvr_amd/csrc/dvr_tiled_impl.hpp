@@ -278,7 +278,7 @@ __device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, float*
 // kinds compiled in: bit0 TF, bit1 volume, bit2 position (camera / step);
 // the runtime flags select among them.
 // ---------------------------------------------------------------------------
-template <int FILTER, int TFM, int BLEND, int K, int GM, class Sink>
+template <int FILTER, int TFM, int BLEND, int K, int GM, class Sink, bool SPLIT = false>
 __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const Ray& r, int n, f4 adj_acc,
                                               const float* ck, int ckS, int cap, int ckEvery, Sink& sink,
                                               bool wantTf,
@@ -362,65 +362,74 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 }
             }
         }
-        // walk the segment backwards
-#pragma unroll
-        for (int j = K - 1; j >= 0; --j) {
-            if (j < cnt) {
-                const int i = i0 + j;
-                const float t = r.t0 + float(i) * r.step;
-                const f4 s = tf_eval<TFM>(T, dv[j]);
-                f4 adj_s;
-                float adj_st;
-                if constexpr (kKeepAK) blend_adjoint_ak<BLEND>(tin[j], s, r.step, al[j], kp[j], adj_acc, adj_s, adj_st);
-                else blend_adjoint_t<BLEND>(tin[j], s, r.step, adj_acc, adj_s, adj_st);
-                float adj_d;
-                if (wantTf) adj_d = tf_adjoint<TFM, CT_TF>(T, dv[j], adj_s, sink);
-                else {
-                    SinkNone none;
-                    adj_d = tf_adjoint<TFM, false>(T, dv[j], adj_s, none);
-                }
-                if constexpr (CT_VOL || CT_POS) {
-                    if (wantVol || wantPos) {
-                        const f3 p = sample_pos(r, t);
-                        f3 adj_p = mk3(0.f, 0.f, 0.f);
-                        if constexpr (FILTER == 0) {
-                            if (wantVol) {
-                                int gx, gy, gz;
-                                nearest_index(p, V.X, V.Y, V.Z, gx, gy, gz);
-                                const int goff = gx * avx + gy * avy + gz * avz;
-                                if (goff != nLast) {
-                                    if (nLast >= 0 && nAcc != 0.f) gatomic_add(avol, unsigned(nLast), nAcc);
-                                    nLast = goff;
-                                    nAcc = 0.f;
-                                }
-                                nAcc += adj_d;
+        // walk the segment backwards; with SPLIT, a segment that is full on every active lane
+        // of the wave (all but a ray's last) runs without a per-sample predicate (a second
+        // copy of the unrolled walk)
+        auto walk = [&](int j) __attribute__((always_inline)) {
+            const int i = i0 + j;
+            const float t = r.t0 + float(i) * r.step;
+            const f4 s = tf_eval<TFM>(T, dv[j]);
+            f4 adj_s;
+            float adj_st;
+            if constexpr (kKeepAK) blend_adjoint_ak<BLEND>(tin[j], s, r.step, al[j], kp[j], adj_acc, adj_s, adj_st);
+            else blend_adjoint_t<BLEND>(tin[j], s, r.step, adj_acc, adj_s, adj_st);
+            float adj_d;
+            if (wantTf) adj_d = tf_adjoint<TFM, CT_TF>(T, dv[j], adj_s, sink);
+            else {
+                SinkNone none;
+                adj_d = tf_adjoint<TFM, false>(T, dv[j], adj_s, none);
+            }
+            if constexpr (CT_VOL || CT_POS) {
+                if (wantVol || wantPos) {
+                    const f3 p = sample_pos(r, t);
+                    f3 adj_p = mk3(0.f, 0.f, 0.f);
+                    if constexpr (FILTER == 0) {
+                        if (wantVol) {
+                            int gx, gy, gz;
+                            nearest_index(p, V.X, V.Y, V.Z, gx, gy, gz);
+                            const int goff = gx * avx + gy * avy + gz * avz;
+                            if (goff != nLast) {
+                                if (nLast >= 0 && nAcc != 0.f) gatomic_add(avol, unsigned(nLast), nAcc);
+                                nLast = goff;
+                                nAcc = 0.f;
                             }
-                        } else {
-                            const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
-                            if (ix != cc.cx || iy != cc.cy || iz != cc.cz) {
-                                if (wantVol && wantPos) corner_move<true, true>(cc, V, avol, avx, avy, avz, ix, iy, iz);
-                                else if (wantVol) corner_move<true, false>(cc, V, avol, avx, avy, avz, ix, iy, iz);
-                                else corner_move<false, true>(cc, V, avol, avx, avy, avz, ix, iy, iz);
-                            }
-                            const float fx = p.x - float(ix), fy = p.y - float(iy), fz = p.z - float(iz);
-                            float w[8];
-                            if (wantPos) tri_adjoint<true>(cc.d, fx, fy, fz, adj_d, w, adj_p);
-                            else tri_adjoint<false>(cc.d, fx, fy, fz, adj_d, w, adj_p);
-                            if (wantVol) {
-#pragma unroll
-                                for (int k = 0; k < 8; ++k) cc.w[k] += w[k];
-                            }
+                            nAcc += adj_d;
                         }
-                        if (wantPos) {
-                            const f3 adj_w = mul3(adj_p, r.invVoxel);
-                            adj_start = add3(adj_start, adj_w);
-                            adj_dir = add3(adj_dir, scl3(adj_w, t));
-                            adj_step += dot3(adj_w, r.dir) * float(i);
+                    } else {
+                        const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
+                        if (ix != cc.cx || iy != cc.cy || iz != cc.cz) {
+                            if (wantVol && wantPos) corner_move<true, true>(cc, V, avol, avx, avy, avz, ix, iy, iz);
+                            else if (wantVol) corner_move<true, false>(cc, V, avol, avx, avy, avz, ix, iy, iz);
+                            else corner_move<false, true>(cc, V, avol, avx, avy, avz, ix, iy, iz);
+                        }
+                        const float fx = p.x - float(ix), fy = p.y - float(iy), fz = p.z - float(iz);
+                        float w[8];
+                        if (wantPos) tri_adjoint<true>(cc.d, fx, fy, fz, adj_d, w, adj_p);
+                        else tri_adjoint<false>(cc.d, fx, fy, fz, adj_d, w, adj_p);
+                        if (wantVol) {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) cc.w[k] += w[k];
                         }
                     }
+                    if (wantPos) {
+                        const f3 adj_w = mul3(adj_p, r.invVoxel);
+                        adj_start = add3(adj_start, adj_w);
+                        adj_dir = add3(adj_dir, scl3(adj_w, t));
+                        adj_step += dot3(adj_w, r.dir) * float(i);
+                    }
                 }
-                adj_step += adj_st;
             }
+            adj_step += adj_st;
+        };
+        bool full = false;
+        if constexpr (SPLIT) full = __all(cnt == K);
+        if (full) {
+#pragma unroll
+            for (int j = K - 1; j >= 0; --j) walk(j);
+        } else {
+#pragma unroll
+            for (int j = K - 1; j >= 0; --j)
+                if (j < cnt) walk(j);
         }
     }
     if constexpr (CT_VOL) {
