@@ -361,20 +361,22 @@ dvr::AdjointParams check_adjoint(const Checked& c, const dvr_outputs* fwd, const
 }
 
 // ---------------------------------------------------------------------------
-// workspace: grow-only device buffer per device for checkpoint scratch
+// workspace: grow-only device buffers per device (slot 0: checkpoint and other
+// scratch, slot 1: the density store of the fused / replaying adjoint)
 // ---------------------------------------------------------------------------
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
+constexpr int kWsSlots = 2;
 std::mutex g_ws_mutex;
 std::vector<Workspace> g_ws;
 
-void* workspace(int device, size_t bytes, hipStream_t stream)
+void* workspace(int device, size_t bytes, hipStream_t stream, int slot = 0)
 {
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if ((int)g_ws.size() <= device) g_ws.resize(device + 1);
-    Workspace& w = g_ws[device];
+    if ((int)g_ws.size() <= device * kWsSlots + slot) g_ws.resize((device + 1) * kWsSlots);
+    Workspace& w = g_ws[device * kWsSlots + slot];
     if (w.bytes >= bytes) return w.ptr;
     if (w.ptr) {
         // earlier work on this stream may still read the old buffer
@@ -417,6 +419,40 @@ int checkpoint_cap(const Checked& c, int K)
 
 int default_k(const Checked&) { return 16; }
 
+// Density store of the checkpointed adjoint: samples per ray it holds (the
+// longest ray when box and step are uniform values, else 1024; longer rays
+// re-interpolate), and its size; stores above kMaxDensityStore bytes are not
+// used by the library's own scratch.
+constexpr double kMaxDensityStore = 2.0 * 1024 * 1024 * 1024;
+int density_cap(const Checked& c)
+{
+    double n = 1024.0;
+    if (!c.P.st && !c.P.bsize) {
+        const double d = std::sqrt(double(c.P.bsizeU[0]) * c.P.bsizeU[0] + double(c.P.bsizeU[1]) * c.P.bsizeU[1] +
+                                   double(c.P.bsizeU[2]) * c.P.bsizeU[2]);
+        n = std::ceil(d / double(c.P.stU)) + 2.0;
+    }
+    return (int)std::max(1.0, std::min(n, double(1 << 24)));
+}
+int64_t density_elements(const Checked& c)
+{
+    return (int64_t)dvr::tiled_num_tiles(c.P) * density_cap(c) * dvr::kBlock;
+}
+// the caller's density buffer, validated (float32, contiguous, on the device, large enough)
+float* density_buffer(const dvr_tensor& t, const Checked& c)
+{
+    int64_t numel = 1;
+    for (int d = 0; d < t.ndim; ++d) numel *= t.sizes[d];
+    if (t.dtype != DVR_FLOAT32 || t.device != c.device)
+        fail(DVR_ERR_INVALID_ARGUMENT, "densities must be a float32 tensor on the render device");
+    if (numel < density_elements(c))
+        fail(DVR_ERR_INVALID_ARGUMENT, "densities needs %lld elements, has %lld", (long long)density_elements(c),
+             (long long)numel);
+    if (t.ndim > 0 && t.strides[t.ndim - 1] != 1 && numel > 1)
+        fail(DVR_ERR_INVALID_ARGUMENT, "densities must be contiguous");
+    return (float*)t.data;
+}
+
 // Checkpoint interval I in {8, 16, 32, 64}: register segments of K = min(I, 32)
 // samples, a checkpoint every I / K segments.
 dvr::CkptParams make_ckpt(const Checked& c, int interval)
@@ -454,6 +490,9 @@ void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forwar
         if (numel < need) fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints needs %lld elements, has %lld", (long long)need, (long long)numel);
         if (ck.ndim > 0 && ck.strides[ck.ndim - 1] != 1 && numel > 1)
             fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints must be contiguous");
+        if (given(opt->densities)) density_buffer(opt->densities, c);
+    } else if (opt && given(opt->densities)) {
+        fail(DVR_ERR_INVALID_ARGUMENT, "densities are written together with checkpoints (checkpoint_interval > 0)");
     }
     if (c.preshaded) {
         check_hip(dvr::launch_forward_preshaded(c.P, c.blend, s), "forward (pre-shaded) launch");
@@ -464,6 +503,10 @@ void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forwar
         if (K > 0) {
             CK = make_ckpt(c, K);
             CK.buf = (float*)opt->checkpoints.data;
+            if (given(opt->densities)) {
+                CK.ds = density_buffer(opt->densities, c);
+                CK.dsCap = density_cap(c);
+            }
         }
         check_hip(dvr::launch_forward_tiled(c.P, CK, c.filter, c.tfm, c.blend, s), "forward (tiled) launch");
     }
@@ -506,8 +549,18 @@ void do_adjoint(const dvr_inputs* inputs, const dvr_outputs* fwd, const dvr_tens
             fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints do not match these inputs and checkpoint_interval");
         CK.buf = (float*)ck.data;
         fromFwd = true;
+        if (given(opt->densities)) {  // written by the forward launch that wrote the checkpoints
+            CK.ds = density_buffer(opt->densities, c);
+            CK.dsCap = density_cap(c);
+        }
     } else {
+        if (opt && given(opt->densities))
+            fail(DVR_ERR_INVALID_ARGUMENT, "densities are read together with the checkpoints of the same forward");
         CK.buf = (float*)workspace(c.device, size_t(need) * sizeof(float), s);
+        if (double(density_elements(c)) * sizeof(float) <= kMaxDensityStore) {  // the replaying forward writes them
+            CK.ds = (float*)workspace(c.device, size_t(density_elements(c)) * sizeof(float), s, 1);
+            CK.dsCap = density_cap(c);
+        }
     }
     dvr::LossParams L;
     std::memset(&L, 0, sizeof(L));
@@ -551,6 +604,10 @@ void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* lo
     hipStream_t s = (hipStream_t)stream;
     const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * CK.cap * dvr::kBlock;
     CK.buf = (float*)workspace(c.device, size_t(need) * sizeof(float), s);
+    if (double(density_elements(c)) * sizeof(float) <= kMaxDensityStore) {
+        CK.ds = (float*)workspace(c.device, size_t(density_elements(c)) * sizeof(float), s, 1);
+        CK.dsCap = density_cap(c);
+    }
     check_hip(dvr::launch_adjoint_tiled(c.P, A, CK, L, true, false, c.filter, c.tfm, c.blend, s), "fused launch");
 }
 
@@ -902,6 +959,16 @@ int64_t dvr_checkpoint_buffer_elements(const dvr_inputs* inputs, int32_t checkpo
     guarded([&] {
         Checked c = check_inputs(inputs);
         r = (int64_t)dvr::tiled_num_tiles(c.P) * checkpoint_cap(c, checkpoint_interval) * dvr::kBlock;
+    });
+    return r;
+}
+
+int64_t dvr_density_buffer_elements(const dvr_inputs* inputs)
+{
+    int64_t r = -1;
+    guarded([&] {
+        Checked c = check_inputs(inputs);
+        r = density_elements(c);
     });
     return r;
 }

@@ -63,6 +63,8 @@ struct CkptParams {
     int K;        // register segment length (8, 16, 32)
     int cap;      // checkpoints per ray
     int every;    // checkpoint interval in segments (1 or 2): interval = K*every
+    float* ds;    // density store [tile][dsCap][kBlock] of the forward march, or NULL
+    int dsCap;    // samples per ray it holds
 };
 
 // Per-pixel loss whose gradient the fused forward+adjoint kernel forms in

@@ -460,9 +460,11 @@ __global__ __launch_bounds__(kBlock) void k_forward_tiled(RenderParams P, CkptPa
     constexpr int kC = tf_cols<TFM>();
     const Tf T{lds, P.R, kC, kC, 1};
     float* ck = CK.buf ? CK.buf + (size_t(tc.tile) * CK.cap) * kBlock + threadIdx.x : nullptr;
+    float* ds = CK.ds ? CK.ds + (size_t(tc.tile) * CK.dsCap) * kBlock + threadIdx.x : nullptr;
     Acc acc;
     int n;
-    march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, CK.K * CK.every, CK.cap);
+    march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, CK.K * CK.every, CK.cap, ds, kBlock,
+                                      CK.dsCap);
     float* o = P.color + tc.b * P.colSb + tc.y * P.colSy + tc.x * P.colSx;
     o[0] = acc.r; o[P.colSc] = acc.g; o[2 * P.colSc] = acc.b; o[3 * P.colSc] = acc.A;
     P.term[tc.b * P.termSb + tc.y * P.termSy + tc.x * P.termSx] = n;
@@ -497,11 +499,15 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
         constexpr int kC = tf_cols<TFM>();
         const Tf T{s_tf, P.R, kC, kC, 1};
         float* ck = CK.buf + (size_t(tc.tile) * CK.cap) * kBlock + threadIdx.x;
+        // density store: written by this kernel's forward march (fused, or replayed without
+        // checkpoints) or by the forward launch that wrote the checkpoints
+        float* ds = CK.ds ? CK.ds + (size_t(tc.tile) * CK.dsCap) * kBlock + threadIdx.x : nullptr;
         int n;
         f4 adj_acc;
         if (fused) {
             Acc acc;
-            march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, K * CK.every, CK.cap);
+            march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, -1, acc, n, ck, kBlock, K * CK.every, CK.cap, ds,
+                                              kBlock, CK.dsCap);
             float* o = P.color + b * P.colSb + y * P.colSy + x * P.colSx;
             o[0] = acc.r; o[P.colSc] = acc.g; o[2 * P.colSc] = acc.b; o[3 * P.colSc] = acc.A;
             P.term[b * P.termSb + y * P.termSy + x * P.termSx] = n;
@@ -523,16 +529,18 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
             if (!ckFromFwd) {
                 Acc acc;
                 int n2;
-                march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, n, acc, n2, ck, kBlock, K * CK.every, CK.cap);
+                march_forward<FILTER, TFM, BLEND>(V, T, r, P.earlyOut, n, acc, n2, ck, kBlock, K * CK.every, CK.cap,
+                                                  ds, kBlock, CK.dsCap);
             }
         }
         f3 adj_start = mk3(0.f, 0.f, 0.f), adj_dir = mk3(0.f, 0.f, 0.f);
         float adj_step = 0.0f;
         const bool wantVol = A.avol != nullptr;
         const bool wantPos = A.ars != nullptr || A.ast != nullptr;
+        const bool stored = ds != nullptr && __all(n <= CK.dsCap);  // uniform over the wave
         march_reverse<FILTER, TFM, BLEND, K, GM>(V, T, r, n, adj_acc, ck, kBlock, CK.cap, CK.every, sink, wantTf, wantVol,
                                                  wantPos, wantVol ? A.avol + b * A.avSb : nullptr, A.avSx, A.avSy,
-                                                 A.avSz, adj_start, adj_dir, adj_step);
+                                                 A.avSz, adj_start, adj_dir, adj_step, stored ? ds : nullptr, kBlock);
         if (A.ast) {
             float* o = A.ast + b * A.astSb + y * A.astSy + x * A.astSx;
             if (A.stepBroadcast) atomicAdd(o, adj_step);

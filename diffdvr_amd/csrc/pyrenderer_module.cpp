@@ -529,7 +529,7 @@ PYBIND11_MODULE(_pyrenderer, m)
 
     m.def("render_forward",
           [](const RendererInputs& inputs, RendererOutputs& outputs, const std::string& path, int checkpoint_interval,
-             py::object checkpoints) {
+             py::object checkpoints, py::object densities) {
               PackedInputs p = pack_inputs(inputs);
               dvr_outputs o = pack_outputs(outputs);
               dvr_forward_options opt;
@@ -537,13 +537,15 @@ PYBIND11_MODULE(_pyrenderer, m)
               opt.path = parse_path(path);
               opt.checkpoint_interval = checkpoint_interval;
               opt.checkpoints = to_tensor(checkpoints);
+              opt.densities = to_tensor(densities);
               check(dvr_render_forward_ex(&p.in, &o, &opt, current_stream(p.in.volume.device)));
           },
           py::arg("inputs"), py::arg("outputs"), py::arg("path") = "auto", py::arg("checkpoint_interval") = 0,
-          py::arg("checkpoints") = py::none());
+          py::arg("checkpoints") = py::none(), py::arg("densities") = py::none());
     m.def("render_adjoint",
           [](const RendererInputs& inputs, const RendererOutputs& fwd, py::object adj_color, AdjointOutputs& adj,
-             const std::string& method, const std::string& path, int checkpoint_interval, py::object checkpoints) {
+             const std::string& method, const std::string& path, int checkpoint_interval, py::object checkpoints,
+             py::object densities) {
               PackedInputs p = pack_inputs(inputs);
               dvr_outputs o = pack_outputs(fwd);
               dvr_tensor ac = to_tensor(adj_color);
@@ -554,11 +556,12 @@ PYBIND11_MODULE(_pyrenderer, m)
               opt.path = parse_path(path);
               opt.checkpoint_interval = checkpoint_interval;
               opt.checkpoints = to_tensor(checkpoints);
+              opt.densities = to_tensor(densities);
               check(dvr_render_adjoint_ex(&p.in, &o, &ac, &ao, &opt, current_stream(p.in.volume.device)));
           },
           py::arg("inputs"), py::arg("outputs_from_forward"), py::arg("adj_color"), py::arg("adj_outputs"),
           py::arg("method") = "auto", py::arg("path") = "auto", py::arg("checkpoint_interval") = 0,
-          py::arg("checkpoints") = py::none());
+          py::arg("checkpoints") = py::none(), py::arg("densities") = py::none());
     m.def("render_fused",
           [](const RendererInputs& inputs, RendererOutputs& outputs, const std::string& loss, py::object target,
              AdjointOutputs& adj, py::object loss_out, float scale, int checkpoint_interval) {
@@ -726,6 +729,14 @@ PYBIND11_MODULE(_pyrenderer, m)
               return n;
           },
           py::arg("inputs"), py::arg("checkpoint_interval"));
+    m.def("density_buffer_elements",
+          [](const RendererInputs& inputs) {
+              PackedInputs p = pack_inputs(inputs);
+              const int64_t n = dvr_density_buffer_elements(&p.in);
+              if (n < 0) throw std::runtime_error(dvr_last_error());
+              return n;
+          },
+          py::arg("inputs"));
     m.def("generate_rays",
           [](py::object viewport, float fovY, int W, int H, py::object ray_start, py::object ray_dir) {
               dvr_tensor v = to_tensor(viewport), s = to_tensor(ray_start), d = to_tensor(ray_dir);

@@ -7,8 +7,9 @@ and pytests/diffdvr/utils.py (``fibonacci_sphere``, ``make_real3`` ...).
 
 Differences to the reference:
   * the adjoint is checkpointed: the forward launch writes the transmittance
-    every ``checkpoint_interval`` samples into a buffer saved for backward, and
-    the backward launch recomputes each segment from it (no blend inversion);
+    every ``checkpoint_interval`` samples and every sample's density (up to
+    2 GiB) into buffers saved for backward, and the backward launch recomputes
+    each segment from them (no blend inversion);
     ``adjoint_method="inversion"`` selects the reference's scheme;
   * ``gradient_method="forward"`` traces the derivatives in launch slices of
     eight indices (any D) instead of one dual number of width D; the
@@ -25,6 +26,9 @@ import torch.nn.functional as F
 
 from . import pyrenderer
 from .scenes import fibonacci_sphere  # noqa: F401  (re-exported, pytests/diffdvr/utils.py:76-89)
+
+# bytes of the per-sample density store kept from the forward for the backward pass
+_MAX_DENSITY_STORE = 2 * 1024 ** 3
 
 renderer_dtype_torch = torch.float32
 renderer_dtype_np = np.float32
@@ -155,14 +159,19 @@ class _RendererFunction(torch.autograd.Function):
         # pre-shaded volumes have the reference's plain / inversion path only
         preshaded = inputs.volume_filter_mode == pyrenderer.VolumeFilterMode.Preshaded
         K = checkpoint_interval if (adjoint_method == "checkpoint" and not preshaded) else 0
-        checkpoints = None
+        checkpoints = densities = None
         if K > 0:
             checkpoints = torch.empty(pyrenderer.Renderer.checkpoint_buffer_elements(inputs, K),
                                       dtype=torch.float32, device=volume.device)
+            # every sample's density for the backward pass (read instead of re-interpolated),
+            # when the store stays within _MAX_DENSITY_STORE bytes
+            nd = pyrenderer.Renderer.density_buffer_elements(inputs)
+            if 4 * nd <= _MAX_DENSITY_STORE:
+                densities = torch.empty(nd, dtype=torch.float32, device=volume.device)
         with Timer(profiling is not None, volume.is_cuda) as timer:
             if K > 0:
                 pyrenderer.Renderer.render_forward(inputs, outputs, path="tiled", checkpoint_interval=K,
-                                                   checkpoints=checkpoints)
+                                                   checkpoints=checkpoints, densities=densities)
             else:
                 pyrenderer.Renderer.render_forward(inputs, outputs, path="plain")
         if profiling is not None:
@@ -170,7 +179,8 @@ class _RendererFunction(torch.autograd.Function):
             profiling.forward_bytes += size_of_tensors(camera_matrix, camera_ray_start, camera_ray_dir, tf, volume,
                                                        color, term, checkpoints)
         ctx.checkpoint_interval = K
-        ctx.save_for_backward(camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, color, term, checkpoints)
+        ctx.save_for_backward(camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, color, term, checkpoints,
+                              densities)
         return color
 
     @staticmethod
@@ -262,7 +272,8 @@ class _RendererFunction(torch.autograd.Function):
     def backward(ctx, grad_output_color):
         if not ctx.use_adjoint_method:
             return _RendererFunction._backward_forward_mode(ctx, grad_output_color)
-        camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, color, term, checkpoints = ctx.saved_tensors
+        camera_matrix, camera_ray_start, camera_ray_dir, tf, volume, color, term, checkpoints, densities = \
+            ctx.saved_tensors
         optimize_camera, optimize_tf, optimize_volume, tf_delayed_accumulation = ctx.flags
         inputs = ctx.inputs
         if inputs.camera_mode == pyrenderer.CameraMode.RayStartDir:
@@ -294,7 +305,7 @@ class _RendererFunction(torch.autograd.Function):
             if ctx.checkpoint_interval > 0:
                 pyrenderer.Renderer.render_adjoint(inputs, outputs, grad_output_color.contiguous(), adj,
                                                    method="checkpoint", checkpoint_interval=ctx.checkpoint_interval,
-                                                   checkpoints=checkpoints)
+                                                   checkpoints=checkpoints, densities=densities)
             else:
                 pyrenderer.Renderer.render_adjoint(inputs, outputs, grad_output_color.contiguous(), adj,
                                                    method="inversion", path="plain")

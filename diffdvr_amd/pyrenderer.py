@@ -77,21 +77,28 @@ class Renderer:
 
     @staticmethod
     def render_forward(inputs: RendererInputs, outputs: RendererOutputs, *, path: str = "auto",
-                       checkpoint_interval: int = 0, checkpoints: torch.Tensor | None = None) -> None:
-        """Simple forward rendering without gradients (renderer/renderer.h:159-161)."""
-        _n.render_forward(inputs, outputs, path, checkpoint_interval, checkpoints)
+                       checkpoint_interval: int = 0, checkpoints: torch.Tensor | None = None,
+                       densities: torch.Tensor | None = None) -> None:
+        """Simple forward rendering without gradients (renderer/renderer.h:159-161).
+
+        With checkpoints, ``densities`` (>= density_buffer_elements(inputs) floats) receives every
+        sample's density for a later checkpointed adjoint."""
+        _n.render_forward(inputs, outputs, path, checkpoint_interval, checkpoints, densities)
 
     @staticmethod
     def render_adjoint(inputs: RendererInputs, outputs_from_forward: RendererOutputs, adj_color: torch.Tensor,
                        adj_outputs: AdjointOutputs, *, method: str = "auto", path: str = "auto",
-                       checkpoint_interval: int = 0, checkpoints: torch.Tensor | None = None) -> None:
+                       checkpoint_interval: int = 0, checkpoints: torch.Tensor | None = None,
+                       densities: torch.Tensor | None = None) -> None:
         """Adjoint rendering; gradients are ACCUMULATED into adj_outputs (renderer/renderer.h:264-268).
 
         method: "auto"/"checkpoint" (alpha checkpoints, segment recompute) or
         "inversion" (the reference's blend inversion, plain launch structure).
+        densities: those the forward call with the same checkpoints wrote (read instead of
+        re-interpolating the volume).
         """
         _n.render_adjoint(inputs, outputs_from_forward, adj_color, adj_outputs, method, path,
-                          checkpoint_interval, checkpoints)
+                          checkpoint_interval, checkpoints, densities)
 
     @staticmethod
     def render_fused(inputs: RendererInputs, outputs: RendererOutputs, loss: str, target: torch.Tensor,
@@ -107,6 +114,10 @@ class Renderer:
     @staticmethod
     def checkpoint_buffer_elements(inputs: RendererInputs, checkpoint_interval: int) -> int:
         return _n.checkpoint_buffer_elements(inputs, checkpoint_interval)
+
+    @staticmethod
+    def density_buffer_elements(inputs: RendererInputs) -> int:
+        return _n.density_buffer_elements(inputs)
 
     @staticmethod
     def render_forward_gradients(inputs: RendererInputs, differences_settings: ForwardDifferencesSettings,

@@ -122,6 +122,8 @@ typedef struct dvr_forward_options {
     int32_t path;                /* DVR_PATH_* */
     int32_t checkpoint_interval; /* K; 0 = no checkpoints written */
     dvr_tensor checkpoints;      /* float32 buffer of >= dvr_checkpoint_buffer_elements() */
+    dvr_tensor densities;        /* optional, with checkpoints: float32 buffer of >= dvr_density_buffer_elements()
+                                    that receives every sample's density for the checkpointed adjoint */
 } dvr_forward_options;
 
 typedef struct dvr_adjoint_options {
@@ -129,6 +131,9 @@ typedef struct dvr_adjoint_options {
     int32_t method;              /* DVR_ADJOINT_* */
     int32_t checkpoint_interval; /* K in {8,16,32,64}; 0 = default (16) */
     dvr_tensor checkpoints;      /* optional: written by dvr_render_forward_ex with the same K */
+    dvr_tensor densities;        /* optional, with checkpoints: the densities that same forward call wrote; the
+                                    reverse march reads them instead of re-interpolating the volume (without
+                                    checkpoints the library uses a scratch store of its own when it fits 2 GiB) */
 } dvr_adjoint_options;
 
 /* ---- library ---------------------------------------------------------- */
@@ -184,6 +189,10 @@ dvr_status dvr_render_fused(const dvr_inputs* inputs, dvr_outputs* outputs, cons
 /* Number of float32 elements a checkpoint buffer needs for these inputs
  * and interval K (0 if K == 0). */
 int64_t dvr_checkpoint_buffer_elements(const dvr_inputs* inputs, int32_t checkpoint_interval);
+/* Elements of the density buffer of dvr_render_forward_ex / dvr_render_adjoint_ex (one float per
+ * sample of the longest ray, per pixel of every 16x16 tile); -1 on invalid inputs.  No reference
+ * counterpart. */
+int64_t dvr_density_buffer_elements(const dvr_inputs* inputs);
 
 /* ---- forward-mode differentiation ---------------------------------------- */
 

@@ -133,7 +133,8 @@ def test_validation_messages(mutate, code, msg):
 
 
 class FwdOptions(ctypes.Structure):
-    _fields_ = [("path", ctypes.c_int32), ("checkpoint_interval", ctypes.c_int32), ("checkpoints", Tensor)]
+    _fields_ = [("path", ctypes.c_int32), ("checkpoint_interval", ctypes.c_int32), ("checkpoints", Tensor),
+                ("densities", Tensor)]
 
 
 @pytest.mark.parametrize("path,K", [(2, 0), (0, 16)])

@@ -176,6 +176,45 @@ def test_forward_checkpoints_handoff():
     compare(g1, want, 1e-4)
 
 
+@pytest.mark.parametrize("filter,step_tensor", [(1, False), (0, False), (1, True)])
+def test_forward_densities_handoff(filter, step_tensor):
+    """The forward launch stores every sample's density with its checkpoints; the adjoint reads them
+    (densities handed over), re-interpolates (checkpoints only), or replays the forward into the
+    library's own store (neither): the same gradients."""
+    rng = np.random.default_rng(37 + filter)
+    s = camera_scene(rng, 33, 29, B=2, filter=filter, tf_mode=1, camera=2, smooth=True, X=20, Y=20, Z=20,
+                     step=0.01)
+    if step_tensor:
+        s["step"] = np.full((2, 29, 33), 0.01) + rng.uniform(-0.002, 0.002, (2, 29, 33))
+    inp = gpu_inputs(s)
+    B, H, W = 2, 29, 33
+    K = 16
+    ck = torch.full((pr.Renderer.checkpoint_buffer_elements(inp, K),), float("nan"), device=dev())
+    nd = pr.Renderer.density_buffer_elements(inp)
+    assert nd >= B * H * W
+    ds = torch.full((nd,), float("nan"), device=dev())
+    color = torch.empty(B, H, W, 4, device=dev())
+    term = torch.empty(B, H, W, dtype=torch.int32, device=dev())
+    pr.Renderer.render_forward(inp, pr.RendererOutputs(color, term), path="tiled", checkpoint_interval=K,
+                               checkpoints=ck, densities=ds)
+    adj_color = t32(rng.standard_normal((B, H, W, 4)))
+    grads = ("tf", "volume", "camera") if filter == 1 else ("tf", "volume")
+    results = []
+    for kw in (dict(checkpoints=ck, densities=ds), dict(checkpoints=ck), {}):
+        a, g = make_adj(s, grads)
+        pr.Renderer.render_adjoint(inp, pr.RendererOutputs(color, term), adj_color, a, checkpoint_interval=K, **kw)
+        results.append(g)
+    torch.cuda.synchronize()
+    for g in results[1:]:
+        for k in g:
+            assert norm_err(results[0][k].cpu().numpy(), g[k].cpu().numpy()) < 1e-6, k
+    with pytest.raises(RuntimeError, match="densities needs"):
+        pr.Renderer.render_forward(inp, pr.RendererOutputs(color, term), path="tiled", checkpoint_interval=K,
+                                   checkpoints=ck, densities=ds[:10])
+    with pytest.raises(RuntimeError, match="together with checkpoints"):
+        pr.Renderer.render_forward(inp, pr.RendererOutputs(color, term), path="tiled", densities=ds)
+
+
 def test_broadcast_gradient_outputs():
     rng = np.random.default_rng(41)
     s = camera_scene(rng, 17, 15, B=1, tf_mode=1, camera=0, smooth=True, X=16, Y=16, Z=16)
