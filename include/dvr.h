@@ -138,6 +138,17 @@ typedef struct dvr_adjoint_options {
 
 /* ---- library ---------------------------------------------------------- */
 
+/*
+ * Library scratch: the checkpointed adjoint without caller checkpoints, fused
+ * rendering, and the TF tables of the forward-mode entry points use device
+ * buffers the library keeps per device (grown on demand: a call that needs a
+ * larger one synchronises the device, frees the old buffer and allocates).
+ * These calls on one device are ordered by their stream; overlapping calls
+ * on different streams of one device race on the scratch.  Entry points given
+ * every buffer by the caller (checkpoints, densities) and batch plans, which
+ * own their scratch, have no such restriction.
+ */
+
 /* Thread-local message of the last failed call ("" if none). */
 const char* dvr_last_error(void);
 /* 0: this library computes in float32 (bindings/bindings.cpp:143-146). */
