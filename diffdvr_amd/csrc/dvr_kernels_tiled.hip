@@ -15,6 +15,13 @@ DVR_DECL_K(1, 1)
 #undef DVR_DECL_K
 #undef DVR_DECL_INST
 
+size_t occupancy_lds(size_t need, int wgPerCu)
+{
+    if (wgPerCu < 2) return need;
+    const size_t l = size_t(kLdsPerCu) / size_t(wgPerCu + 1) + 512;
+    return need > l ? need : l;
+}
+
 int tiled_num_tiles(const RenderParams& P)
 {
     const int tx = (P.W + kTileW - 1) / kTileW, ty = (P.H + kTileH - 1) / kTileH;
@@ -27,7 +34,7 @@ static hipError_t fwd_tiled_t(const RenderParams& P, const CkptParams& CK, hipSt
     const int tilesX = (P.W + kTileW - 1) / kTileW, tilesY = (P.H + kTileH - 1) / kTileH;
     const int nTiles = tilesX * tilesY * P.B;
     if (nTiles == 0) return hipSuccess;
-    const size_t lds = sizeof(float) * size_t(P.R) * P.C;
+    const size_t lds = occupancy_lds(sizeof(float) * size_t(P.R) * P.C, kTiledWgPerCu);
     hipLaunchKernelGGL((k_forward_tiled<FILTER, TFM, BLEND>), dim3(nTiles), dim3(kBlock), lds, s, P, CK, tilesX,
                        tilesX * tilesY, nTiles);
     return hipGetLastError();

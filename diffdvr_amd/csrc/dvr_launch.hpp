@@ -27,6 +27,17 @@
 
 namespace dvr {
 
+// Occupancy cap of the tiled kernels.  A workgroup asks for at least
+// kLdsPerCu / (wgPerCu + 1) + 512 bytes of dynamic LDS, so no more than
+// wgPerCu of them share a CU.  The tiled kernels run at two 4-wave
+// workgroups per CU: more rays in flight evict each other's cells from the
+// 32 KB L1 faster than they hide latency (C2, 8 views 256x256 over 128^3:
+// forward with checkpoints 0.93 -> 0.59 ms, checkpointed adjoint
+// 0.72 -> 0.65 ms against the register-limited occupancy).
+constexpr int kLdsPerCu = 160 * 1024;  // gfx950
+constexpr int kTiledWgPerCu = 2;
+size_t occupancy_lds(size_t need, int wgPerCu);
+
 // dvr_kernels_plain.hip
 hipError_t launch_forward_plain(const RenderParams& P, int filter, int tfm, int blend, hipStream_t s);
 hipError_t launch_adjoint_plain(const RenderParams& P, const AdjointParams& A, int filter, int tfm, int blend,
