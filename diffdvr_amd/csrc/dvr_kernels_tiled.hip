@@ -34,7 +34,7 @@ static hipError_t fwd_tiled_t(const RenderParams& P, const CkptParams& CK, hipSt
     const int tilesX = (P.W + kTileW - 1) / kTileW, tilesY = (P.H + kTileH - 1) / kTileH;
     const int nTiles = tilesX * tilesY * P.B;
     if (nTiles == 0) return hipSuccess;
-    const size_t lds = occupancy_lds(sizeof(float) * size_t(P.R) * P.C, kTiledWgPerCu);
+    const size_t lds = occupancy_lds(sizeof(float) * size_t(P.R) * P.C, tiled_wg_per_cu(TFM));
     hipLaunchKernelGGL((k_forward_tiled<FILTER, TFM, BLEND>), dim3(nTiles), dim3(kBlock), lds, s, P, CK, tilesX,
                        tilesX * tilesY, nTiles);
     return hipGetLastError();

@@ -29,13 +29,17 @@ namespace dvr {
 
 // Occupancy cap of the tiled kernels.  A workgroup asks for at least
 // kLdsPerCu / (wgPerCu + 1) + 512 bytes of dynamic LDS, so no more than
-// wgPerCu of them share a CU.  The tiled kernels run at two 4-wave
+// wgPerCu of them share a CU.  The gather-bound instances run at two 4-wave
 // workgroups per CU: more rays in flight evict each other's cells from the
 // 32 KB L1 faster than they hide latency (C2, 8 views 256x256 over 128^3:
 // forward with checkpoints 0.93 -> 0.59 ms, checkpointed adjoint
-// 0.72 -> 0.65 ms against the register-limited occupancy).
+// 0.72 -> 0.65 ms against the register-limited occupancy; volume gradients
+// of 256^3 -9 %).  The Gaussian TF evaluates every component per sample and
+// is arithmetic-bound: it keeps the register-limited occupancy (the cap cost
+// it 3 %).
 constexpr int kLdsPerCu = 160 * 1024;  // gfx950
 constexpr int kTiledWgPerCu = 2;
+constexpr int tiled_wg_per_cu(int tfm) { return tfm == 3 ? 0 : kTiledWgPerCu; }
 size_t occupancy_lds(size_t need, int wgPerCu);
 
 // dvr_kernels_plain.hip

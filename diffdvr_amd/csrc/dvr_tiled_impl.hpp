@@ -593,7 +593,7 @@ static hipError_t adj_tiled_k(const RenderParams& P, const AdjointParams& A, con
     const int RC = P.R * P.C;
     const bool usePriv = (TFM == 3) && A.atf && RC <= 48;  // private slices <= 48 KiB of LDS
     const size_t lds = occupancy_lds(
-        sizeof(float) * (size_t(RC) + (A.atf ? (usePriv ? size_t(RC) * kBlock : size_t(RC)) : 0)), kTiledWgPerCu);
+        sizeof(float) * (size_t(RC) + (A.atf ? (usePriv ? size_t(RC) * kBlock : size_t(RC)) : 0)), tiled_wg_per_cu(TFM));
     hipLaunchKernelGGL((k_adjoint_tiled<FILTER, TFM, BLEND, K, GM>), dim3(nTiles), dim3(kBlock), lds, s, P, A, CK, L,
                        fused ? 1 : 0, ckFromFwd ? 1 : 0, usePriv ? 1 : 0, tilesX, tilesX * tilesY, nTiles);
     return hipGetLastError();
