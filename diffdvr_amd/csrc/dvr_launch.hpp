@@ -38,7 +38,7 @@ namespace dvr {
 // is arithmetic-bound: it keeps the register-limited occupancy (the cap cost
 // it 3 %).
 constexpr int kLdsPerCu = 160 * 1024;  // gfx950
-constexpr int kTiledWgPerCu = 2;
+constexpr int kTiledWgPerCu = 8;
 constexpr int tiled_wg_per_cu(int tfm) { return tfm == 3 ? 0 : kTiledWgPerCu; }
 size_t occupancy_lds(size_t need, int wgPerCu);
 

@@ -7,9 +7,9 @@
 
 namespace dvr {
 
-constexpr int kTileW = 16;        // pixels per workgroup tile (x)
-constexpr int kTileH = 16;        // pixels per workgroup tile (y)
-constexpr int kBlock = 256;       // threads per workgroup = 4 waves of 8x8 rays
+constexpr int kTileW = 8;         // pixels per workgroup tile (x)
+constexpr int kTileH = 8;         // pixels per workgroup tile (y)
+constexpr int kBlock = 64;        // threads per workgroup = one wave of 8x8 rays
 constexpr int kBatchBlock = 64;   // batched persistent kernel: one wave per workgroup, one wave tile per pop
 constexpr int kTfCopies = 8;      // batched kernel: copies of the LDS TF-gradient table, lane & 7 selects one
 constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin

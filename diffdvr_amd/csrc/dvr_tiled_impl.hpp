@@ -1,6 +1,6 @@
-// Tiled kernels for gfx950: each workgroup renders a 16x16 pixel tile of one
-// view as four wave64s of 8x8 rays, so one wave's trilinear gathers touch a
-// compact bundle of nearby cells.  Tiles are ordered so that each XCD owns a
+// Tiled kernels for gfx950: each workgroup is one wave64 rendering an 8x8
+// pixel tile of one view, so the wave's trilinear gathers touch a compact
+// bundle of nearby cells, and a wave that finishes frees its slot at once.  Tiles are ordered so that each XCD owns a
 // contiguous band of them (shared L2).  The transfer function is staged in
 // LDS; its gradient is privatised in LDS and flushed once per workgroup.
 //
@@ -31,9 +31,8 @@ __device__ __forceinline__ TileXY tile_xy(int tilesX, int tilesPerView, int nTil
     t.b = t.tile / tilesPerView;
     const int tl = t.tile - t.b * tilesPerView;
     const int ty = tl / tilesX, tx = tl - ty * tilesX;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    t.x = tx * kTileW + (wave & 1) * 8 + (lane & 7);
-    t.y = ty * kTileH + (wave >> 1) * 8 + (lane >> 3);
+    t.x = tx * kTileW + int(threadIdx.x & 7);
+    t.y = ty * kTileH + int(threadIdx.x >> 3);
     return t;
 }
 

@@ -201,7 +201,7 @@ dvr_status dvr_render_fused(const dvr_inputs* inputs, dvr_outputs* outputs, cons
  * and interval K (0 if K == 0). */
 int64_t dvr_checkpoint_buffer_elements(const dvr_inputs* inputs, int32_t checkpoint_interval);
 /* Elements of the density buffer of dvr_render_forward_ex / dvr_render_adjoint_ex (one float per
- * sample of the longest ray, per pixel of every 16x16 tile); -1 on invalid inputs.  No reference
+ * sample of the longest ray, per pixel of every 8x8 tile); -1 on invalid inputs.  No reference
  * counterpart. */
 int64_t dvr_density_buffer_elements(const dvr_inputs* inputs);
 
