@@ -537,7 +537,10 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
         const bool wantVol = A.avol != nullptr;
         const bool wantPos = A.ars != nullptr || A.ast != nullptr;
         const bool stored = ds != nullptr && __all(n <= CK.dsCap);  // uniform over the wave
-        march_reverse<FILTER, TFM, BLEND, K, GM>(V, T, r, n, adj_acc, ck, kBlock, CK.cap, CK.every, sink, wantTf, wantVol,
+        // the split walk (a second, predicate-free copy for segments full on every lane) pays off
+        // except for the arithmetic-bound Gaussian TF
+        march_reverse<FILTER, TFM, BLEND, K, GM, SinkTiled<TFM>, TFM != 3>(V, T, r, n, adj_acc, ck, kBlock, CK.cap,
+                                                                           CK.every, sink, wantTf, wantVol,
                                                  wantPos, wantVol ? A.avol + b * A.avSb : nullptr, A.avSx, A.avSy,
                                                  A.avSz, adj_start, adj_dir, adj_step, stored ? ds : nullptr, kBlock);
         if (A.ast) {
