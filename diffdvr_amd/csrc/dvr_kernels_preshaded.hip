@@ -161,6 +161,12 @@ __global__ __launch_bounds__(256) void k_adjoint_preshaded(RenderParams P, Adjoi
     }
 }
 
+// The forward march gathers 32 floats per new cell: at most two 256-thread
+// workgroups per CU keep the rays in flight within what L1 holds (C2 scene
+// pre-shaded, 4 x 128^3: 3.04 -> 2.72 ms).  The adjoint is bound by its
+// volume-gradient atomics and keeps the register-limited occupancy.
+constexpr int kPreshadedFwdWgPerCu = 2;
+
 static int preshaded_grid(int total)
 {
     const int want = (total + 255) / 256;
@@ -171,8 +177,9 @@ hipError_t launch_forward_preshaded(const RenderParams& P, int blend, hipStream_
 {
     const int total = P.W * P.H * P.B;
     if (total == 0) return hipSuccess;
-    if (blend == 0) hipLaunchKernelGGL(k_forward_preshaded<0>, dim3(preshaded_grid(total)), dim3(256), 0, s, P, total);
-    else hipLaunchKernelGGL(k_forward_preshaded<1>, dim3(preshaded_grid(total)), dim3(256), 0, s, P, total);
+    const size_t lds = occupancy_lds(0, kPreshadedFwdWgPerCu);
+    if (blend == 0) hipLaunchKernelGGL(k_forward_preshaded<0>, dim3(preshaded_grid(total)), dim3(256), lds, s, P, total);
+    else hipLaunchKernelGGL(k_forward_preshaded<1>, dim3(preshaded_grid(total)), dim3(256), lds, s, P, total);
     return hipGetLastError();
 }
 
