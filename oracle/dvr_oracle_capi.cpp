@@ -78,6 +78,20 @@ template <class T> oracle::Inputs<T> tf_only(int mode, const T* tf, int R, int C
     {                                                                                                          \
         oracle::render_forward<T>(convert<T>(in), color, term, frozen_term, frozen_t0, t0_out);                \
     }                                                                                                          \
+    /* camera rays of every pixel (a2): start, dir B*H*W*3 */                                              \
+    extern "C" void oracle_camera_rays_##SUF(const oracle_inputs* in, T* start, T* dir)                       \
+    {                                                                                                          \
+        const oracle::Inputs<T> c = convert<T>(in);                                                            \
+        for (int b = 0; b < c.B; ++b)                                                                          \
+            for (int y = 0; y < c.H; ++y)                                                                      \
+                for (int x = 0; x < c.W; ++x) {                                                                \
+                    oracle::V3<T> s, d;                                                                        \
+                    oracle::camera_rays<T>(c, x, y, b, s, d);                                                  \
+                    const size_t o = (((size_t)b * c.H + y) * c.W + x) * 3;                                    \
+                    start[o] = s.x; start[o + 1] = s.y; start[o + 2] = s.z;                                    \
+                    dir[o] = d.x; dir[o + 1] = d.y; dir[o + 2] = d.z;                                          \
+                }                                                                                              \
+    }                                                                                                          \
     extern "C" void oracle_kink_census_##SUF(const oracle_inputs* in, const int* term, T eps, T eps_pos,        \
                                              T eps_face, int* out)                                             \
     {                                                                                                          \

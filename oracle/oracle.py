@@ -158,6 +158,15 @@ def forward(scene, dtype=np.float64, frozen_term=None, frozen_t0=None, return_t0
     return (color, term, t0) if return_t0 else (color, term)
 
 
+def camera_rays(scene, dtype=np.float64):
+    """The scene's camera rays of every pixel (renderer_camera.cuh:12-61) -> start, dir (B,H,W,3)."""
+    p = _Prepared(scene, dtype)
+    start = np.zeros((p.B, p.H, p.W, 3), dtype=p.dtype)
+    d = np.zeros_like(start)
+    getattr(lib(), f"oracle_camera_rays_{_suffix(dtype)}")(ctypes.byref(p.s), _ptr(start), _ptr(d))
+    return start, d
+
+
 def forward_gradients(scene, D, d_step=-1, d_start=(-1, -1, -1), d_dir=(-1, -1, -1), d_tf=None, dtype=np.float64,
                       frozen_term=None, frozen_t0=None):
     """Oracle forward-mode derivatives -> color (B,H,W,4), term (B,H,W), grad (B,H,W,D,4).
@@ -457,3 +466,69 @@ def log_mse_backward(log_x, log_y, grad):
 def log_l1_backward(log_x, log_y, grad):
     with np.errstate(divide="ignore"):
         return grad / -np.expm1(log_y - log_x), grad / -np.expm1(log_x - log_y)
+
+
+# ---------------------------------------------------------------------------
+# camera viewports (renderer/camera.cpp:127-198, orientation tables :70-85),
+# numpy float64; complex inputs are carried through unchanged so that the
+# complex-step derivative (Im f(x + ih) / h) gives exact Jacobians
+# ---------------------------------------------------------------------------
+ORIENTATION_UP = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], dtype=np.float64)
+ORIENTATION_PERMUTATION = [(2, -1, -3), (-2, 1, 3), (1, 2, 3), (-1, -2, -3), (-3, -1, 2), (3, 1, -2)]
+ORIENTATION_INVERT_YAW = [True, False, False, True, False, True]
+ORIENTATION_INVERT_PITCH = [False, False, False, False, False, False]
+
+
+def _cross(a, b):
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                     a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
+
+
+def _normalize(v):
+    # sqrt of the sum of squares (analytic: no abs), as torch.nn.functional.normalize away from 0
+    return v / np.sqrt(np.sum(v * v, axis=-1, keepdims=True))
+
+
+def viewport_from_lookat(origin, look_at, up):
+    """B x 3 x 3 rows (eye, right, up) (camera.cpp:127-147): front = normalize(lookAt - origin),
+    right = normalize(front x up), up' = normalize(right x front)."""
+    origin, look_at, up = (np.atleast_2d(np.asarray(a)) for a in (origin, look_at, up))
+    B = max(origin.shape[0], look_at.shape[0], up.shape[0])
+    origin, look_at, up = (np.broadcast_to(a, (B, 3)) for a in (origin, look_at, up))
+    front = _normalize(look_at - origin)
+    right = _normalize(_cross(front, up))
+    up2 = _normalize(_cross(right, front))
+    return np.stack([origin, right, up2], axis=1)
+
+
+def viewport_from_sphere(center, yaw, pitch, distance, orientation):
+    """Viewport of a camera at `distance` from `center` (camera.cpp:149-198): yaw and pitch are
+    negated unless the orientation inverts them; position (cos p cos y, sin p, cos p sin y) * r is
+    permuted / sign-flipped by the orientation's permutation; looks at the center with the
+    orientation's up vector.  center B x 3, yaw / pitch / distance B x 1 (size-1 batches broadcast)."""
+    o = int(orientation)
+    center = np.atleast_2d(np.asarray(center))
+    yaw = np.asarray(yaw).reshape(-1, 1) * (1.0 if ORIENTATION_INVERT_YAW[o] else -1.0)
+    pitch = np.asarray(pitch).reshape(-1, 1) * (1.0 if ORIENTATION_INVERT_PITCH[o] else -1.0)
+    distance = np.asarray(distance).reshape(-1, 1)
+    pos = [np.cos(pitch) * np.cos(yaw) * distance, np.sin(pitch) * distance, np.cos(pitch) * np.sin(yaw) * distance]
+    pos2 = [pos[abs(p) - 1] * (1.0 if p > 0 else -1.0) for p in ORIENTATION_PERMUTATION[o]]
+    B = max(center.shape[0], yaw.shape[0], pitch.shape[0], distance.shape[0])
+    origin = np.broadcast_to(np.concatenate(np.broadcast_arrays(*pos2), axis=1), (B, 3))
+    c = np.broadcast_to(center, (B, 3))
+    return viewport_from_lookat(origin + c, c, np.broadcast_to(ORIENTATION_UP[o], (B, 3)))
+
+
+def viewport_from_sphere_jacobian(center, yaw, pitch, distance, orientation, h=1e-30):
+    """d viewport / d (yaw, pitch, distance): B x 3 x 3 x 3 (last axis: yaw, pitch, distance),
+    by the complex step (exact to rounding for these analytic functions)."""
+    args = [np.asarray(yaw, dtype=np.float64).reshape(-1, 1), np.asarray(pitch, dtype=np.float64).reshape(-1, 1),
+            np.asarray(distance, dtype=np.float64).reshape(-1, 1)]
+    cols = []
+    for k in range(3):
+        a = [x.astype(np.complex128) for x in args]
+        a[k] = a[k] + 1j * h
+        cols.append(np.imag(viewport_from_sphere(np.asarray(center, dtype=np.float64), a[0], a[1], a[2],
+                                                 orientation)) / h)
+    return np.stack(cols, axis=-1)
