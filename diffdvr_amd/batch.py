@@ -180,6 +180,16 @@ class BatchPlan:
     def zero_gradients(self) -> None:
         self._h.zero_gradients(self._stream())
 
+    def kernel_ms(self, zero_gradients: bool = True) -> float:
+        """One run with HIP events around the persistent kernel alone (on the current stream);
+        waits for it and returns that kernel's duration in milliseconds."""
+        return float(self._h.time_kernel(ZERO_GRADIENTS if zero_gradients else 0, self._stream()))
+
+    def freeze(self) -> None:
+        """Fix the plan's layout (scratch buffers, kernel arguments) for graph capture: afterwards
+        set_options may only repeat the current options and autotune raises."""
+        self._h.freeze()
+
     @property
     def options(self) -> dict:
         k, w, g, ri = self._h.options()
@@ -262,6 +272,7 @@ class GraphedSteps:
 
     def __init__(self, plan: BatchPlan, adam: Adam, n: int):
         self.plan, self.adam, self.n = plan, adam, int(n)
+        plan.freeze()   # the graph holds the plan's scratch pointers: no re-layout from now on
         self.graph = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(device=plan.device)
         s.wait_stream(torch.cuda.current_stream(plan.device))

@@ -83,6 +83,7 @@ struct dvr_batch_plan {
     int cus = 0, resident = 0;  // compute units, resident workgroups of the current instance
     int qBegin[dvr::kNumXcd + 1] = {};
     size_t lds = 0;
+    int aggOffset = 0;  // float offset of the volume-gradient chunk table in LDS (0: none)
     // device
     dvr::BView* dViews = nullptr;
     dvr::BVol* dVols = nullptr;
@@ -97,6 +98,7 @@ struct dvr_batch_plan {
     size_t dsFloats = 0;
     int dsCap = 0;
     std::string signature;  // autotune cache key
+    bool frozen = false;    // captured graphs hold the scratch pointers: no re-layout
 
     ~dvr_batch_plan()
     {
@@ -130,6 +132,7 @@ struct dvr_batch_plan {
         P.tileW = waveWidth; P.tileH = 64 / waveWidth;
         P.ldsTf = maxRC; P.usePriv = usePriv ? 1 : 0;
         P.tfCopyStride = (gm && !usePriv) ? copy_stride() : 0;
+        P.ldsAgg = aggOffset;
         return P;
     }
 
@@ -152,6 +155,12 @@ struct dvr_batch_plan {
                           : (copy_stride() > 0 ? size_t(dvr::kTfCopies) * copy_stride() : size_t(maxRC)))
                : 0;
         lds = sizeof(float) * (size_t(maxRC) + ldsG);
+        // volume gradients: the per-wave table of 64-byte chunks follows the TF tables
+        aggOffset = 0;
+        if (gm & 2) {
+            aggOffset = dvr::agg_offset(maxRC, (int)ldsG);
+            lds = sizeof(float) * size_t(aggOffset) + dvr::kAggBytes;
+        }
         int occ = 0;
         check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, 0, lds, nullptr, &occ),
                   "occupancy query");
@@ -200,12 +209,15 @@ struct dvr_batch_plan {
         check_hip(dvr::launch_zero_segments(dSegsAll, n, mx, s), "zero launch");
     }
 
-    void run(int flags, hipStream_t s) const
+    // evBegin / evEnd (optional) are recorded around the persistent kernel alone
+    void run(int flags, hipStream_t s, hipEvent_t evBegin = nullptr, hipEvent_t evEnd = nullptr) const
     {
         zero((flags & DVR_BATCH_ZERO_GRADIENTS) != 0, s);
-        if (nTiles == 0) return;
-        check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, grid, lds, s, nullptr),
-                  "batched launch");
+        if (evBegin) check_hip(hipEventRecord(evBegin, s), "hipEventRecord");
+        if (nTiles > 0)
+            check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, grid, lds, s, nullptr),
+                      "batched launch");
+        if (evEnd) check_hip(hipEventRecord(evEnd, s), "hipEventRecord");
     }
 };
 
@@ -231,6 +243,15 @@ void set_options(dvr_batch_plan* p, const dvr_batch_options* o)
 {
     int K, ww;
     check_options(o, K, ww);
+    if (p->frozen) {
+        const bool same = K == p->K && ww == p->waveWidth && (o ? o->grid : 0) == p->gridReq &&
+                          (o && o->reinterpolate != 0) == p->reinterpolate;
+        if (!same)
+            fail(DVR_ERR_INVALID_ARGUMENT,
+                 "the plan is frozen (a captured graph holds its scratch buffers and kernel arguments): "
+                 "its options cannot change; create a new plan");
+        return;
+    }
     p->K = K;
     p->waveWidth = ww;
     p->gridReq = o ? o->grid : 0;
@@ -446,6 +467,40 @@ dvr_status dvr_batch_plan_run(dvr_batch_plan* plan, int32_t flags, void* stream)
     });
 }
 
+dvr_status dvr_batch_plan_time_kernel(dvr_batch_plan* plan, int32_t flags, void* stream, float* kernel_ms)
+{
+    return guarded([&] {
+        if (!plan || !kernel_ms) fail(DVR_ERR_INVALID_ARGUMENT, "NULL argument");
+        DeviceGuard g(plan->device);
+        hipStream_t s = (hipStream_t)stream;
+        hipEvent_t e0, e1;
+        check_hip(hipEventCreate(&e0), "hipEventCreate");
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            fail(DVR_ERR_HIP, "hipEventCreate failed");
+        }
+        try {
+            plan->run(flags, s, e0, e1);
+            check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+            check_hip(hipEventElapsedTime(kernel_ms, e0, e1), "hipEventElapsedTime");
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
+dvr_status dvr_batch_plan_freeze(dvr_batch_plan* plan)
+{
+    return guarded([&] {
+        if (!plan) fail(DVR_ERR_INVALID_ARGUMENT, "plan is NULL");
+        plan->frozen = true;
+    });
+}
+
 dvr_status dvr_batch_plan_zero_gradients(dvr_batch_plan* plan, void* stream)
 {
     return guarded([&] {
@@ -480,6 +535,10 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
 {
     return guarded([&] {
         if (!plan) fail(DVR_ERR_INVALID_ARGUMENT, "plan is NULL");
+        if (plan->frozen)
+            fail(DVR_ERR_INVALID_ARGUMENT,
+                 "the plan is frozen (a captured graph holds its scratch buffers and kernel arguments): "
+                 "autotune before capturing");
         DeviceGuard g(plan->device);
         hipStream_t s = (hipStream_t)stream;
         const int gridReq = plan->gridReq;

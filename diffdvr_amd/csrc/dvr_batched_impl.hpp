@@ -76,6 +76,17 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
     // this lane's density store: sample i at ds[i * kBatchBlock]
     float* ds = ADJ && B.ds ? B.ds + size_t(blockIdx.x) * size_t(B.dsCap) * kBatchBlock + threadIdx.x : nullptr;
     const bool priv = B.usePriv != 0;
+    // volume gradients: the wave's corner values are summed per 64-byte chunk in LDS and flushed
+    // at the end of every tile (each tile belongs to one view, hence one volume)
+    constexpr bool kVolInst = (GM & 2) != 0;
+    using VS = std::conditional_t<kVolInst, VolAgg, VolDirect>;
+    VS vs;
+    const bool aggOn = kVolInst && B.wantVol && B.ldsAgg > 0;
+    if constexpr (kVolInst) {
+        if (aggOn) vs.init(lds + B.ldsAgg);
+    } else {
+        vs.p = nullptr;
+    }
     int qoff = 0;
     int curTf = -1, curRC = 0;
     bool curWantTf = false;
@@ -113,6 +124,11 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
             __syncthreads();
         }
         const BVol vol = B.vols[V.vol];
+        if constexpr (kVolInst) {
+            // every lane binds (uniform): the flush hands table entries to all lanes, also to
+            // lanes outside the image
+            if (aggOn && vol.adj != nullptr) vs.bind(vol.adj);
+        }
         const int tl = tile - V.tileBegin;
         const int ty = tl / V.tilesX, tx = tl - ty * V.tilesX;
         const int x = tx * B.tileW + lx, y = ty * B.tileH + ly;
@@ -172,14 +188,14 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
                 }
                 f3 adj_start = mk3(0.f, 0.f, 0.f), adj_dir = mk3(0.f, 0.f, 0.f);
                 float adj_step = 0.0f;
-                const bool wantVol = B.wantVol && vol.adj != nullptr;
+                const bool wantVol = aggOn && vol.adj != nullptr;
                 // the store holds every sample of every active ray of the wave (else re-interpolate;
                 // the choice is uniform over the wave, so the segment loop does not diverge)
                 const bool stored = ds != nullptr && __all(n <= B.dsCap);
-                march_reverse<FILTER, TFM, BLEND, K, GM, SinkTiled<TFM>, true>(VV, T, r, n, adj_acc, ck, kBatchBlock,
-                                                                               B.ckCap, B.ckEvery, sink,
-                                                         curWantTf,
-                                                         wantVol, wantCam, vol.adj, vol.Y * vol.Z, vol.Z, 1,
+                march_reverse<FILTER, TFM, BLEND, K, GM, SinkTiled<TFM>, true, VS>(VV, T, r, n, adj_acc, ck,
+                                                                                   kBatchBlock, B.ckCap, B.ckEvery,
+                                                                                   sink, curWantTf,
+                                                         wantVol, wantCam, vs, vol.Y * vol.Z, vol.Z, 1,
                                                          adj_start, adj_dir, adj_step, stored ? ds : nullptr,
                                                          kBatchBlock);
                 if constexpr ((GM & 4) != 0) {
@@ -205,6 +221,9 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
             }
         }
         if constexpr (ADJ) {
+            if constexpr (kVolInst) {
+                if (aggOn && vol.adj != nullptr) vs.flush();  // uniform: one view per tile
+            }
             if (curWantTf) sink.template finish<TFM == 2 ? 5 : 4>();
             if constexpr ((GM & 4) != 0) {
                 if (wantCam) {  // uniform: one view per tile

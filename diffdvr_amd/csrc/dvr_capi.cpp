@@ -393,6 +393,7 @@ void* workspace(int device, size_t bytes, hipStream_t stream, int slot = 0)
 }
 
 
+static_assert(dvr::kMaxTfLds == DVR_MAX_TILED_TF_ENTRIES, "the LDS TF limit is part of the C-ABI");
 bool tiled_ok(const Checked& c) { return !c.preshaded && (int64_t)c.P.R * c.P.C <= dvr::kMaxTfLds; }
 
 void no_preshaded(const Checked& c, const char* what)

@@ -14,6 +14,13 @@ constexpr int kBatchBlock = 64;   // batched persistent kernel: one wave per wor
 constexpr int kTfCopies = 8;      // batched kernel: copies of the LDS TF-gradient table, lane & 7 selects one
 constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kMaxTfLds = 4096;   // max R*C floats of a TF staged in LDS by the tiled kernels
+// LDS table of the volume-gradient aggregation (one per wave): 2^kAggLog2 slots
+// of one 64-byte chunk each, plus tags, slot list and counter
+constexpr int kAggLog2 = 8;
+constexpr int kAggSlots = 1 << kAggLog2;
+constexpr size_t kAggBytes = size_t(kAggSlots) * (4 + 4 + 16 * 4) + 16;
+// float offset of the table in dynamic LDS after used0 + used1 floats (16-byte aligned)
+constexpr int agg_offset(int used0, int used1) { return (used0 + used1 + 3) & ~3; }
 constexpr int kMaxSamples = 1 << 24;  // hard cap on samples per ray (termination guarantee)
 
 struct RenderParams {
@@ -135,6 +142,7 @@ struct BatchParams {
     int ldsTf;                  // floats of LDS per TF table (max R*C over the TFs)
     int usePriv;                // Gaussian TF gradient in per-thread LDS slices
     int tfCopyStride;           // floats between the kTfCopies TF-gradient tables (0: one table)
+    int ldsAgg;                 // float offset of the volume-gradient chunk table in LDS (0: none)
 };
 
 // One segment of a zero-fill list (gradients, counters) cleared by a kernel

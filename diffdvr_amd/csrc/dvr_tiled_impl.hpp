@@ -13,6 +13,7 @@
 // Forward and reverse semantics: renderer/renderer_kernels.cuh:22-90 and
 // :324-444 (step-size adjoint accumulated over all samples, see dvr_device.hpp).
 #pragma once
+#include <type_traits>
 #include "dvr_device.hpp"
 #include "dvr_launch.hpp"
 
@@ -206,24 +207,156 @@ __device__ __forceinline__ unsigned corner_off(const CellOffsets& o, int k)
     return o.base + ((k & 4) ? o.dx : 0u) + ((k & 2) ? o.dy : 0u) + ((k & 1) ? o.dz : 0u);
 }
 
+// ---------------------------------------------------------------------------
+// Volume-gradient sinks of the reverse walk.  Both take element offsets from
+// the gradient base; zero values are dropped.
+//
+// VolDirect: one device-memory float atomic per corner (the reference's
+// flush, renderer_interpolation.cuh:299-324).
+//
+// VolAgg: a device-memory float atomic costs one 64-byte request whether one
+// lane or sixteen lanes of a wave instruction address that 64-byte chunk
+// (measured on MI355X: ~21 G requests/s for lanes on distinct chunks, so
+// ~21 G floats/s scattered and ~330 G floats/s for 16-lane groups on one
+// chunk).  The 64 rays of a wave, walked over a few samples, touch ~7 corner
+// values per distinct 64-byte chunk, so the wave's corner gradients are first
+// summed in an LDS table of chunks (open addressing on the chunk index:
+// ds_cmpst claims a slot, ds_add_f32 adds the value) and the table is flushed
+// with one 16-lane atomic group per occupied chunk.  A chunk that finds no free
+// slot within kAggProbe probes goes straight to device memory.  The table
+// belongs to one wave (one-wave workgroups) and to one gradient tensor: it is
+// flushed when it is half full, at the end of every tile and before its
+// gradient base changes.  The flush runs on whichever lanes are active (the
+// work is spread by rank among them), so it may be called from divergent code.
+// ---------------------------------------------------------------------------
+constexpr int kAggProbe = 8;
+constexpr unsigned kAggFlushAt = kAggSlots / 2;
+constexpr unsigned kAggEmpty = 0xFFFFFFFFu;
+
+struct VolDirect {
+    static constexpr bool kAgg = false;
+    float* p;
+    __device__ __forceinline__ void add(unsigned off, float v)
+    {
+        if (v != 0.f) gatomic_add(p, off, v);
+    }
+    __device__ __forceinline__ void add2(unsigned o0, float v0, unsigned o1, float v1)
+    {
+        add(o0, v0);
+        add(o1, v1);
+    }
+    __device__ __forceinline__ void maybe_flush() {}
+};
+
+struct VolAgg {
+    static constexpr bool kAgg = true;
+    unsigned* tags;   // [kAggSlots] chunk index, kAggEmpty = free
+    unsigned* list;   // [kAggSlots] occupied slots in claim order
+    unsigned* ctr;    // number of occupied slots
+    float* vals;      // [kAggSlots][16]
+    float* abase;     // gradient base rounded down to 64 bytes (never dereferenced below the base)
+    unsigned boff;    // floats from abase to the gradient base
+
+    // every thread of the (one-wave) workgroup; lds is 16-byte aligned, kAggBytes long
+    __device__ __forceinline__ void init(void* lds)
+    {
+        tags = (unsigned*)lds;
+        list = tags + kAggSlots;
+        ctr = list + kAggSlots;
+        vals = (float*)(ctr + 4);
+        for (int e = threadIdx.x; e < kAggSlots; e += blockDim.x) tags[e] = kAggEmpty;
+        for (int e = threadIdx.x; e < kAggSlots * 16; e += blockDim.x) vals[e] = 0.f;
+        if (threadIdx.x == 0) *ctr = 0u;
+    }
+    // the table must be empty (flushed) when the base changes
+    __device__ __forceinline__ void bind(float* g)
+    {
+        const unsigned long long a = (unsigned long long)g;
+        abase = (float*)(a & ~63ull);
+        boff = unsigned(a & 63ull) >> 2;
+    }
+    __device__ __forceinline__ unsigned claim(unsigned q)
+    {
+        unsigned s = (q * 0x9E3779B1u) >> (32 - kAggLog2);
+        unsigned old = atomicCAS(&tags[s], kAggEmpty, q);
+        // linear probing only for the (rare) lanes whose slot holds another chunk
+        for (int p = 1; p < kAggProbe && old != kAggEmpty && old != q; ++p) {
+            s = (s + 1) & (kAggSlots - 1);
+            old = atomicCAS(&tags[s], kAggEmpty, q);
+        }
+        if (old == kAggEmpty) list[atomicAdd(ctr, 1u)] = s;
+        else if (old != q) s = kAggEmpty;
+        return s;
+    }
+    __device__ __forceinline__ void put(unsigned s, unsigned e, float v)
+    {
+        if (s != kAggEmpty) atomicAdd(&vals[s * 16 + (e & 15)], v);
+        else gatomic_add(abase, e, v);
+    }
+    __device__ __forceinline__ void add(unsigned off, float v)
+    {
+        if (v == 0.f) return;
+        const unsigned e = boff + off;
+        put(claim(e >> 4), e, v);
+    }
+    // two values, usually z-neighbours in one chunk (then one claim)
+    __device__ __forceinline__ void add2(unsigned o0, float v0, unsigned o1, float v1)
+    {
+        const unsigned e0 = boff + o0, e1 = boff + o1;
+        const bool same = (e0 >> 4) == (e1 >> 4);
+        const bool n0 = v0 != 0.f, n1 = v1 != 0.f;
+        unsigned s0 = kAggEmpty, s1 = kAggEmpty;
+        if (n0 || (n1 && same)) s0 = claim(e0 >> 4);
+        if (n1 && !same) s1 = claim(e1 >> 4);
+        if (same) s1 = s0;
+        if (n0) put(s0, e0, v0);
+        if (n1) put(s1, e1, v1);
+    }
+    __device__ __forceinline__ void flush()
+    {
+        const unsigned long long act = __ballot(1);
+        const unsigned cnt = unsigned(__popcll(act));
+        const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(act >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(act), 0u));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const unsigned n = *ctr;
+        for (unsigned e = rank; e < n * 16u; e += cnt) {
+            const unsigned s = list[e >> 4], j = e & 15u;
+            const float v = vals[s * 16 + j];
+            if (v != 0.f) gatomic_add(abase, tags[s] * 16u + j, v);
+            vals[s * 16 + j] = 0.f;
+        }
+        for (unsigned e = rank; e < n; e += cnt) tags[list[e]] = kAggEmpty;
+        if (rank == 0) *ctr = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+    __device__ __forceinline__ void maybe_flush()
+    {
+        if (*ctr > kAggFlushAt) flush();
+    }
+};
+
 struct CornerCache {
     int cx, cy, cz;   // current cell, kInvalidCell = none
     float w[8];       // accumulated corner gradients (volume adjoint)
     float d[8];       // corner densities (position adjoint)
 };
 
-// Move across the face normal to axis bit B, towards +1 (UP) or -1.
+// Move across the face normal to axis bit B, towards +1 (UP) or -1: the four
+// leaving corners' offsets and values go to lo/lv[0..3] (pairs along z unless
+// B is z), the shared corners' state moves over, the fresh corners start empty.
 template <int B, bool UP, bool VOL, bool POS>
-__device__ __forceinline__ void corner_shift(CornerCache& c, const Vol& V, float* avol, const CellOffsets& oldG,
-                                             const CellOffsets& newV)
+__device__ __forceinline__ void corner_shift(CornerCache& c, const Vol& V, const CellOffsets& oldG,
+                                             const CellOffsets& newV, unsigned (&lo)[8], float (&lv)[8])
 {
+    int m = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         if (k & B) continue;
         const int f = UP ? k : (k | B);   // leaves; takes the shared corner's state
         const int s = UP ? (k | B) : k;   // the new cell's fresh corner
         if constexpr (VOL) {
-            if (c.w[f] != 0.f) gatomic_add(avol, corner_off(oldG, f), c.w[f]);
+            lo[m] = corner_off(oldG, f);
+            lv[m] = c.w[f];
             c.w[f] = c.w[s];
             c.w[s] = 0.f;
         }
@@ -231,42 +364,66 @@ __device__ __forceinline__ void corner_shift(CornerCache& c, const Vol& V, float
             c.d[f] = c.d[s];
             c.d[s] = gld(V.p, corner_off(newV, s));
         }
+        ++m;
     }
 }
 
-template <bool VOL, bool POS>
-__device__ __forceinline__ void corner_flush_all(CornerCache& c, const Vol& V, float* avol, int avx, int avy, int avz)
+template <bool VOL, bool POS, class VS>
+__device__ __forceinline__ void corner_flush_all(CornerCache& c, const Vol& V, VS& vs, int avx, int avy, int avz)
 {
     if constexpr (VOL) {
         if (c.cx != kInvalidCell) {
             const CellOffsets o = cell_offsets(c.cx, c.cy, c.cz, V.X, V.Y, V.Z, avx, avy, avz);
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (c.w[k] != 0.f) gatomic_add(avol, corner_off(o, k), c.w[k]);
+            for (int k = 0; k < 8; k += 2) vs.add2(corner_off(o, k), c.w[k], corner_off(o, k + 1), c.w[k + 1]);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) c.w[k] = 0.f;
     }
 }
 
-template <bool VOL, bool POS>
-__device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, float* avol, int avx, int avy, int avz, int ix,
+// The leaving corners are collected by the (divergent) direction cases and
+// handed to the sink once, after them: the sink's insertion code then runs
+// once per cell change instead of once per direction case.
+template <bool VOL, bool POS, class VS>
+__device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs, int avx, int avy, int avz, int ix,
                                             int iy, int iz)
 {
     const int ddx = ix - c.cx, ddy = iy - c.cy, ddz = iz - c.cz;
-    const bool unit = c.cx != kInvalidCell && (abs(ddx) + abs(ddy) + abs(ddz)) == 1;
+    const bool valid = c.cx != kInvalidCell;
+    const bool unit = valid && (abs(ddx) + abs(ddy) + abs(ddz)) == 1;
+    unsigned lo[8];
+    float lv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { lo[k] = 0u; lv[k] = 0.f; }
     if (unit) {
         const CellOffsets oldG = cell_offsets(c.cx, c.cy, c.cz, V.X, V.Y, V.Z, avx, avy, avz);
         const CellOffsets newV = cell_offsets(ix, iy, iz, V.X, V.Y, V.Z, V.sx, V.sy, V.sz);
-        if (ddx > 0) corner_shift<4, true, VOL, POS>(c, V, avol, oldG, newV);
-        else if (ddx < 0) corner_shift<4, false, VOL, POS>(c, V, avol, oldG, newV);
-        else if (ddy > 0) corner_shift<2, true, VOL, POS>(c, V, avol, oldG, newV);
-        else if (ddy < 0) corner_shift<2, false, VOL, POS>(c, V, avol, oldG, newV);
-        else if (ddz > 0) corner_shift<1, true, VOL, POS>(c, V, avol, oldG, newV);
-        else corner_shift<1, false, VOL, POS>(c, V, avol, oldG, newV);
+        if (ddx > 0) corner_shift<4, true, VOL, POS>(c, V, oldG, newV, lo, lv);
+        else if (ddx < 0) corner_shift<4, false, VOL, POS>(c, V, oldG, newV, lo, lv);
+        else if (ddy > 0) corner_shift<2, true, VOL, POS>(c, V, oldG, newV, lo, lv);
+        else if (ddy < 0) corner_shift<2, false, VOL, POS>(c, V, oldG, newV, lo, lv);
+        else if (ddz > 0) corner_shift<1, true, VOL, POS>(c, V, oldG, newV, lo, lv);
+        else corner_shift<1, false, VOL, POS>(c, V, oldG, newV, lo, lv);
     } else {
-        corner_flush_all<VOL, POS>(c, V, avol, avx, avy, avz);
+        if constexpr (VOL) {
+            if (valid) {
+                const CellOffsets o = cell_offsets(c.cx, c.cy, c.cz, V.X, V.Y, V.Z, avx, avy, avz);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) { lo[k] = corner_off(o, k); lv[k] = c.w[k]; }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) c.w[k] = 0.f;
+        }
         if constexpr (POS) cell_load(V, ix, iy, iz, c.d);
+    }
+    if constexpr (VOL) {
+        vs.add2(lo[0], lv[0], lo[1], lv[1]);
+        vs.add2(lo[2], lv[2], lo[3], lv[3]);
+        if (!unit) {
+            vs.add2(lo[4], lv[4], lo[5], lv[5]);
+            vs.add2(lo[6], lv[6], lo[7], lv[7]);
+        }
     }
     c.cx = ix; c.cy = iy; c.cz = iz;
 }
@@ -277,11 +434,11 @@ __device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, float*
 // kinds compiled in: bit0 TF, bit1 volume, bit2 position (camera / step);
 // the runtime flags select among them.
 // ---------------------------------------------------------------------------
-template <int FILTER, int TFM, int BLEND, int K, int GM, class Sink, bool SPLIT = false>
+template <int FILTER, int TFM, int BLEND, int K, int GM, class Sink, bool SPLIT = false, class VS = VolDirect>
 __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const Ray& r, int n, f4 adj_acc,
                                               const float* ck, int ckS, int cap, int ckEvery, Sink& sink,
                                               bool wantTf,
-                                              bool wantVol, bool wantPos, float* avol, int avx, int avy, int avz,
+                                              bool wantVol, bool wantPos, VS& vs, int avx, int avy, int avz,
                                               f3& adj_start, f3& adj_dir, float& adj_step,
                                               const float* dsr = nullptr, int dsS = 0)
 {
@@ -388,7 +545,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                             nearest_index(p, V.X, V.Y, V.Z, gx, gy, gz);
                             const int goff = gx * avx + gy * avy + gz * avz;
                             if (goff != nLast) {
-                                if (nLast >= 0 && nAcc != 0.f) gatomic_add(avol, unsigned(nLast), nAcc);
+                                if (nLast >= 0) vs.add(unsigned(nLast), nAcc);
                                 nLast = goff;
                                 nAcc = 0.f;
                             }
@@ -397,9 +554,9 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                     } else {
                         const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
                         if (ix != cc.cx || iy != cc.cy || iz != cc.cz) {
-                            if (wantVol && wantPos) corner_move<true, true>(cc, V, avol, avx, avy, avz, ix, iy, iz);
-                            else if (wantVol) corner_move<true, false>(cc, V, avol, avx, avy, avz, ix, iy, iz);
-                            else corner_move<false, true>(cc, V, avol, avx, avy, avz, ix, iy, iz);
+                            if (wantVol && wantPos) corner_move<true, true>(cc, V, vs, avx, avy, avz, ix, iy, iz);
+                            else if (wantVol) corner_move<true, false>(cc, V, vs, avx, avy, avz, ix, iy, iz);
+                            else corner_move<false, true>(cc, V, vs, avx, avy, avz, ix, iy, iz);
                         }
                         const float fx = p.x - float(ix), fy = p.y - float(iy), fz = p.z - float(iz);
                         float w[8];
@@ -419,6 +576,9 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 }
             }
             adj_step += adj_st;
+            if constexpr (VS::kAgg && CT_VOL) {
+                if ((j & 3) == 0 && wantVol) vs.maybe_flush();
+            }
         };
         bool full = false;
         if constexpr (SPLIT) full = __all(cnt == K);
@@ -434,9 +594,9 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
     if constexpr (CT_VOL) {
         if (wantVol) {
             if constexpr (FILTER == 0) {
-                if (nLast >= 0 && nAcc != 0.f) gatomic_add(avol, unsigned(nLast), nAcc);
+                if (nLast >= 0) vs.add(unsigned(nLast), nAcc);
             } else {
-                corner_flush_all<true, false>(cc, V, avol, avx, avy, avz);
+                corner_flush_all<true, false>(cc, V, vs, avx, avy, avz);
             }
         }
     }
@@ -480,10 +640,19 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
     float* s_g = lds + RC;
     const TileXY tc = tile_xy(tilesX, tilesPerView, nTiles);
     const bool wantTf = A.atf != nullptr;
+    const int ng = wantTf ? (usePriv ? RC * kBlock : RC) : 0;
     stage_tf(P, tc.b, s_tf);
-    if (wantTf) {
-        const int ng = usePriv ? RC * kBlock : RC;
-        for (int e = threadIdx.x; e < ng; e += kBlock) s_g[e] = 0.0f;
+    for (int e = threadIdx.x; e < ng; e += kBlock) s_g[e] = 0.0f;
+    // volume gradient: the wave's corner values are summed per 64-byte chunk in LDS
+    constexpr bool kVolInst = (GM & 2) != 0;
+    using VS = std::conditional_t<kVolInst, VolAgg, VolDirect>;
+    VS vs;
+    const bool aggOn = kVolInst && A.avol != nullptr;
+    if constexpr (kVolInst) {
+        if (aggOn) {
+            vs.init(lds + agg_offset(RC, ng));
+            vs.bind(A.avol + tc.b * A.avSb);
+        }
     }
     __syncthreads();
 
@@ -539,9 +708,10 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
         const bool stored = ds != nullptr && __all(n <= CK.dsCap);  // uniform over the wave
         // the split walk (a second, predicate-free copy for segments full on every lane) pays off
         // except for the arithmetic-bound Gaussian TF
-        march_reverse<FILTER, TFM, BLEND, K, GM, SinkTiled<TFM>, TFM != 3>(V, T, r, n, adj_acc, ck, kBlock, CK.cap,
-                                                                           CK.every, sink, wantTf, wantVol,
-                                                 wantPos, wantVol ? A.avol + b * A.avSb : nullptr, A.avSx, A.avSy,
+        if constexpr (!kVolInst) vs.p = nullptr;
+        march_reverse<FILTER, TFM, BLEND, K, GM, SinkTiled<TFM>, TFM != 3, VS>(V, T, r, n, adj_acc, ck, kBlock, CK.cap,
+                                                                               CK.every, sink, wantTf, wantVol,
+                                                 wantPos, vs, A.avSx, A.avSy,
                                                  A.avSz, adj_start, adj_dir, adj_step, stored ? ds : nullptr, kBlock);
         if (A.ast) {
             float* o = A.ast + b * A.astSb + y * A.astSy + x * A.astSx;
@@ -559,6 +729,9 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
                 od[0] = adj_dir.x; od[A.ardSc] = adj_dir.y; od[2 * A.ardSc] = adj_dir.z;
             }
         }
+    }
+    if constexpr (kVolInst) {
+        if (aggOn) vs.flush();
     }
     if (wantTf) sink.template finish<TFM == 2 ? 5 : 4>();
     if (fused && L.lossOut) {
@@ -594,8 +767,10 @@ static hipError_t adj_tiled_k(const RenderParams& P, const AdjointParams& A, con
     if (nTiles == 0) return hipSuccess;
     const int RC = P.R * P.C;
     const bool usePriv = (TFM == 3) && A.atf && RC <= 48;  // private slices <= 48 KiB of LDS
-    const size_t lds = occupancy_lds(
-        sizeof(float) * (size_t(RC) + (A.atf ? (usePriv ? size_t(RC) * kBlock : size_t(RC)) : 0)), tiled_wg_per_cu(TFM));
+    const int ng = A.atf ? (usePriv ? RC * kBlock : RC) : 0;
+    const size_t need = ((GM & 2) && A.avol) ? sizeof(float) * size_t(agg_offset(RC, ng)) + kAggBytes
+                                             : sizeof(float) * (size_t(RC) + size_t(ng));
+    const size_t lds = occupancy_lds(need, tiled_wg_per_cu(TFM));
     hipLaunchKernelGGL((k_adjoint_tiled<FILTER, TFM, BLEND, K, GM>), dim3(nTiles), dim3(kBlock), lds, s, P, A, CK, L,
                        fused ? 1 : 0, ckFromFwd ? 1 : 0, usePriv ? 1 : 0, tilesX, tilesX * tilesY, nTiles);
     return hipGetLastError();

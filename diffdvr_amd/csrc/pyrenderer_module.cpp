@@ -384,6 +384,7 @@ struct AdamHandle {
 
 PYBIND11_MODULE(_pyrenderer, m)
 {
+    m.attr("MAX_TILED_TF_ENTRIES") = DVR_MAX_TILED_TF_ENTRIES;
     m.doc() = "MI355X (gfx950) differentiable direct volume renderer: pyrenderer API over libdvr_hip";
     m.def("use_double_precision", []() { return dvr_use_double_precision(); },
           "Returns 0: this library computes in single precision.");
@@ -759,6 +760,12 @@ PYBIND11_MODULE(_pyrenderer, m)
         .def("zero_gradients", [](BatchPlanHandle& h, uintptr_t stream) {
             check(dvr_batch_plan_zero_gradients(h.plan, reinterpret_cast<void*>(stream)));
         })
+        .def("time_kernel", [](BatchPlanHandle& h, int flags, uintptr_t stream) {
+            float ms = 0.0f;
+            check(dvr_batch_plan_time_kernel(h.plan, flags, reinterpret_cast<void*>(stream), &ms));
+            return ms;
+        })
+        .def("freeze", [](BatchPlanHandle& h) { check(dvr_batch_plan_freeze(h.plan)); })
         .def("options", &plan_options)
         .def("set_options", [](BatchPlanHandle& h, int K, int ww, int grid, bool reinterpolate) {
             dvr_batch_options o{K, ww, grid, reinterpolate ? 1 : 0};

@@ -159,6 +159,8 @@ class _RendererFunction(torch.autograd.Function):
         # pre-shaded volumes have the reference's plain / inversion path only
         preshaded = inputs.volume_filter_mode == pyrenderer.VolumeFilterMode.Preshaded
         K = checkpoint_interval if (adjoint_method == "checkpoint" and not preshaded) else 0
+        if K > 0 and tf.shape[-2] * tf.shape[-1] > pyrenderer.MAX_TILED_TF_ENTRIES:
+            K = 0   # TF too large for the LDS of the tiled kernels: plain forward + inversion adjoint
         checkpoints = densities = None
         if K > 0:
             checkpoints = torch.empty(pyrenderer.Renderer.checkpoint_buffer_elements(inputs, K),

@@ -32,6 +32,9 @@ import torch.nn.functional as F
 
 from ._native import native as _n
 
+# R*C of the largest TF the tiled / checkpointed / batched kernels stage in LDS (include/dvr.h)
+MAX_TILED_TF_ENTRIES = int(_n.MAX_TILED_TF_ENTRIES)
+
 real3 = _n.real3
 real4 = _n.real4
 int2 = _n.int2

@@ -57,6 +57,9 @@ typedef int32_t dvr_status;
 #define DVR_TF_GAUSSIAN 3
 #define DVR_TF_PRESHADED 5     /* the TF passes the pre-shaded colour on; any TF tensor size */
 #define DVR_TF_GAUSSIAN_LOG 6  /* log of the Gaussian TF: pre-shading only, not a render mode (as the reference) */
+/* R*C entries of the largest TF the tiled, checkpointed, fused, forward-mode and batched kernels
+ * stage in LDS; larger TFs render on the plain path (forward and inversion adjoint) only. */
+#define DVR_MAX_TILED_TF_ENTRIES 4096
 #define DVR_BLEND_BEER_LAMBERT 0
 #define DVR_BLEND_ALPHA 1
 
@@ -419,6 +422,14 @@ dvr_status dvr_batch_plan_destroy(dvr_batch_plan* plan);
 dvr_status dvr_batch_plan_run(dvr_batch_plan* plan, int32_t flags, void* stream);
 /* Clear the gradient buffers and loss_out with a kernel (no memset). */
 dvr_status dvr_batch_plan_zero_gradients(dvr_batch_plan* plan, void* stream);
+/* Measurement: one run (as dvr_batch_plan_run) with HIP events recorded on `stream` around the
+ * persistent kernel alone; waits for it and writes that kernel's duration.  No counterpart in the
+ * reference (its GPUTimer, bindings/bindings.cpp:99-129, times whatever the caller brackets). */
+dvr_status dvr_batch_plan_time_kernel(dvr_batch_plan* plan, int32_t flags, void* stream, float* kernel_ms);
+/* Freeze the plan's layout: from now on set_options may only repeat the current options and
+ * autotune fails.  Call before capturing runs into a HIP graph: a captured graph holds the
+ * plan's scratch pointers and kernel arguments, which a re-layout would free or change. */
+dvr_status dvr_batch_plan_freeze(dvr_batch_plan* plan);
 dvr_status dvr_batch_plan_get_options(const dvr_batch_plan* plan, dvr_batch_options* options);
 /* Re-layout for new options; may allocate (set-up only, not under capture). */
 dvr_status dvr_batch_plan_set_options(dvr_batch_plan* plan, const dvr_batch_options* options);

@@ -1048,9 +1048,12 @@ inline int kink_samples(const Inputs<T>& in, int x, int y, int b, int n, T eps, 
     for (int i = 0; i < n; ++i) {
         const T t = r.t0 + T(i) * r.step;
         const V3<T> p = sample_pos(r, t);
-        // density and the magnitude its FP32 rounding error scales with
+        // density and the magnitude its FP32 rounding error scales with; `exact`: every
+        // precision computes the same density (a voxel value, or a cell whose corners are all
+        // equal: a + t (b - a) = a), so comparisons of it with 0, 1 or a control point agree
         T d, scale;
         T absorption = T(0);
+        bool exact = false;
         if (in.filter == FilterPreshaded) {
             TriCell<T> cells[4];
             absorption = preshaded_fetch(in, p, cells).w;
@@ -1058,22 +1061,28 @@ inline int kink_samples(const Inputs<T>& in, int x, int y, int b, int n, T eps, 
         } else if (in.filter == FilterNearest) {
             d = fetch_density(in, b, p);
             scale = std::fabs(d);
+            exact = true;
         } else {
             TriCell<T> c;
             trilinear_cell(in, vb, p, c);
             d = trilinear_value(c);
             scale = T(0);
-            for (int k = 0; k < 8; ++k) scale = std::max(scale, std::fabs(c.d[k]));
+            exact = true;
+            for (int k = 0; k < 8; ++k) {
+                scale = std::max(scale, std::fabs(c.d[k]));
+                exact &= c.d[k] == c.d[0];
+            }
         }
         const T band = eps * scale;
         bool kink = false;
         if (in.tf_mode == TFIdentity) {
-            kink = std::fabs(d) < band || std::fabs(d - T(1)) < band;
+            kink = !exact && (std::fabs(d) < band || std::fabs(d - T(1)) < band);
         } else if (in.tf_mode == TFTexture) {
+            // d * R - 0.5 rounds differently per precision even for an exact d
             const T u = d * T(in.R) - T(0.5);
             kink = std::fabs(u - std::round(u)) < band * T(in.R);
         } else if (in.tf_mode == TFLinear) {
-            for (int k = 0; k < in.R; ++k) kink |= std::fabs(d - tf[k * in.C + 4]) < band;
+            for (int k = 0; k < in.R && !exact; ++k) kink |= std::fabs(d - tf[k * in.C + 4]) < band;
         }
         if (in.blend == BlendAlpha) {
             const T a = (in.filter == FilterPreshaded ? absorption : tf_eval(in, b, d).w) * r.step;

@@ -242,12 +242,22 @@ def test_kink_census():
     s = axis_scene(0.6, 1.7, 0.9, 0.1)
     _, t = oracle.forward(s)
     assert np.all(oracle.kink_census(s, t) == 0)        # identity TF, density inside (0, 1)
+    # density on the identity clamp at 1: a uniform volume gives exactly 1 in every precision (no
+    # kink); with y-slices 1, 1, 1, 0.9, 0.9 the middle row (p.y = 2) has density 1 from unequal
+    # corners, the other rows lie in uniform cells (y = 0..1) or at 0.9
     s1 = axis_scene(1.0, 1.7, 0.9, 0.1)
-    assert np.all(oracle.kink_census(s1, t) == t)       # density on the identity clamp at 1
-    # linear TF with a control point exactly at the density: every sample is on the kink
+    assert np.all(oracle.kink_census(s1, t) == 0)
+    s1["volume"][:, :, 3:, :] = 0.9
+    c1 = oracle.kink_census(s1, t)
+    assert np.all(c1[0, 1] == t[0, 1]) and np.all(c1[0, [0, 2]] == 0)
+    # linear TF with a control point exactly at the density 0.6: on the kink where the corners differ
     s2 = dict(s, tf_mode=2, tf=np.array([[[0.1, 0.2, 0.3, 1.0, 0.0], [0.4, 0.5, 0.6, 2.0, 0.6],
                                            [0.7, 0.8, 0.9, 3.0, 1.0]]]))
-    assert np.all(oracle.kink_census(s2, t) == t)
+    assert np.all(oracle.kink_census(s2, t) == 0)
+    s2["volume"] = s2["volume"].copy()
+    s2["volume"][:, :, 3:, :] = 0.7
+    c2 = oracle.kink_census(s2, t)
+    assert np.all(c2[0, 1] == t[0, 1]) and np.all(c2[0, [0, 2]] == 0)
     s2["tf"][0, 1, 4] = 0.61                            # 0.01 away: no kink
     assert np.all(oracle.kink_census(s2, t) == 0)
     # texture TF with R = 5: texel edges where density * 5 - 0.5 is an integer, e.g. 0.5
