@@ -9,14 +9,18 @@ blended march iteration; N_samples = sum of the termination indices of the
 forward pass, and the adjoint processes the same N.
 
 Workloads (synthetic, seeded, identical on every host; diffdvr_amd/scenes.py):
-  c2 (default, the metric's configuration): 128^3 blobs, 64x4 texture TF,
+  c5 (default: the largest single-GPU configuration): 16 volumes {64..256}^3 with
+     4-32 cameras each at 128..512, linear TF per volume, L2, density + TF
+     gradient, HIP Adam, 100 steps captured as one HIP graph; beside it the plain
+     per-view form (reference-structured forward + inversion adjoint per view,
+     then the same Adam step), the FP32 CPU port on sampled views, and the FP64
+     oracle's error on those views
+  c2: 128^3 blobs, 64x4 texture TF,
      8 views 256^2 at latitude 20, step 0.25 voxel, L1 vs a second TF, TF gradient
   c3: 256^3 blobs + noise, identity TF x10, 64 Fibonacci views 256^2, step 0.5
      voxel, L2 vs a second volume, gradient w.r.t. all voxels, K autotuned
   c4: 128^3 blobs, 4-Gaussian TF, 512 cameras of sizes {64..512}^2, L2 vs images
      from perturbed cameras, gradient w.r.t. latitude / longitude / radius
-  c5: 16 volumes {64..256}^3 with 4-32 cameras each at 128..512, linear TF per
-     volume, L2, density + TF gradient, HIP Adam, 100 steps captured as one HIP graph
   fwd: forward mode on the C2 scene, D = 7 (per-pixel step size, ray start, ray
      direction), contracted with the L1 loss gradient
   preshade: TF pre-shading of a 256^3 volume (4-Gaussian TF) and its density
@@ -53,8 +57,9 @@ METRIC = "fwd+adjoint ray-samples/s on 1 MI355X, batched vs plain, at 1e-4 rel g
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "fwd", "preshade"])
-    p.add_argument("--steps", type=int, default=0, help="timed steps (0 = per-workload default)")
+    p.add_argument("--workload", default="c5", choices=["c2", "c3", "c4", "c5", "fwd", "preshade"])
+    p.add_argument("--steps", type=int, default=0,
+                   help="timed steps (0 = per-workload default; c5: rounded up to whole 100-step graphs)")
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--profile", action="store_true")
@@ -506,28 +511,92 @@ def bench_c4(args):
 
 
 # ---------------------------------------------------------------------------
-# C5: multi-volume optimisation loop, 100 steps as one HIP graph
+# C5: multi-volume optimisation loop, 100 steps as one HIP graph (the default
+# workload: the largest single-GPU configuration)
 # ---------------------------------------------------------------------------
+def host_cpu():
+    """CPU model, sockets and logical CPUs of this host (from /proc/cpuinfo)."""
+    model, sockets, logical = "unknown", set(), 0
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k == "model name":
+                model = v.strip()
+                logical += 1
+            elif k == "physical id":
+                sockets.add(v.strip())
+    except OSError:
+        pass
+    return {"cpu_model": model, "sockets": max(1, len(sockets)), "logical_cpus": logical or (os.cpu_count() or 0)}
+
+
+def c5_oracle_scene(vol_np, tf_np, vp_np, W, H, step, fov):
+    import numpy as np
+    return dict(W=W, H=H, filter=1, camera=2, tf_mode=2, blend=0, early_out=1 - 1e-5, fov_y=fov,
+                volume=vol_np[None], box_min=np.full((1, 3), -0.5), box_size=np.ones((1, 3)),
+                step=np.full((1, 1, 1), step), tf=tf_np[None], cam_matrix=vp_np[None])
+
+
+def cpu_baseline_c5(sample, n_all):
+    """The FP32 OpenMP restatement of renderer_cpu.cpp (forward + L2 gradient + inversion
+    adjoint with density and TF gradients) on the sampled views: one warm-up pass, then the
+    median of five."""
+    import numpy as np
+    from oracle import oracle
+
+    # one GPU's share of the host's cores: the box gives each GPU 16, whatever nproc shows
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    scenes32 = [(c5_oracle_scene(*(a.astype(np.float32) for a in (v, t, vp)), W, H, st, fov), tg)
+                for (v, t, vp, W, H, st, fov, tg) in sample]
+
+    def one_pass():
+        n = 0.0
+        for s, tg in scenes32:
+            c, t = oracle.forward(s, dtype=np.float32)
+            adj = (2.0 * (c - tg[None]) / n_all).astype(np.float32)
+            oracle.adjoint(s, c, t, adj, grads=("volume", "tf"), dtype=np.float32)
+            n += float(t.sum())
+        return n
+
+    one_pass()
+    times, n = [], 0.0
+    for _ in range(5):
+        t0 = time.perf_counter()
+        n = one_pass()
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": n / med, "unit": "ray-samples/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sample)} of the workload's views (the first view of each volume in order until "
+                      f"2.5e7 GPU samples and 3 views; {n:.3g} samples per pass): FP32 forward + L2 gradient + inversion "
+                      f"adjoint with density and TF gradients, the OpenMP restatement of renderer_cpu.cpp "
+                      f"(schedule(guided) over pixels, omp atomic scatters); 1 warm-up, median of 5 passes "
+                      f"({med:.2f} s)",
+            **host_cpu(), "threads_note": "16 threads = one GPU's share of the GPU host's cores"}
+
+
 def bench_c5(args):
     import numpy as np
     import torch
+    from diffdvr_amd import pyrenderer as pr
     from diffdvr_amd import scenes
     from diffdvr_amd import batch as bt
 
     dev = torch.device("cuda:0")
-    wl = scenes.workload_c5()
+    wl = scenes.workload_c5(device=dev)
     G = args.graph_steps
-    reps = args.steps or 1
+    reps = max(1, -(-(args.steps or G) // G))
     args.steps = reps * G
     stream = torch.cuda.current_stream()
-    vols = [v.to(dev).contiguous() for v in wl.volumes]
-    tgts = [v.to(dev).contiguous() for v in wl.target_volumes]
+    vols = [v.contiguous() for v in wl.volumes]
+    tgts = [v.contiguous() for v in wl.target_volumes]
     tfs = [t.to(dev).contiguous() for t in wl.tfs]
     vps, sizes, vidx, steps_w = [], [], [], []
     for k, cams in enumerate(wl.cameras):
         vp = cams.viewports(dev)
         for i in range(len(cams.sizes)):
-            vps.append(vp[i])
+            vps.append(vp[i].contiguous())
             sizes.append(cams.sizes[i])
             vidx.append(k)
             steps_w.append(wl.steps[k])
@@ -538,6 +607,7 @@ def bench_c5(args):
                 for i in range(len(vps))]
 
     targets = render_targets(bt, tgts, tfs, views_for(), 2)
+    del tgts
     g_vols = [torch.zeros_like(v) for v in vols]
     g_tfs = [torch.zeros_like(t) for t in tfs]
     views = views_for(targets)
@@ -545,29 +615,212 @@ def bench_c5(args):
     plan = bt.BatchPlan([bt.BatchVolume(v, grad=g) for v, g in zip(vols, g_vols)],
                         [bt.BatchTf(t, g) for t, g in zip(tfs, g_tfs)], views, volume_filter_mode=1, tf_mode=2,
                         blend_mode=0, loss="l2", loss_out=loss,
-                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8)
+                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8,
+                        grid=args.grid)
     tuned = None
-    if not args.checkpoint_interval and not args.wave_width:
+    if not (args.checkpoint_interval or args.wave_width or args.grid):
         tuned = plan.autotune(iterations=1)
-    adam = bt.Adam([(v, g, None, (0.0, 1.0)) for v, g in zip(vols, g_vols)] +
-                   [(t, g, 4, (0.0, 1e3)) for t, g in zip(tfs, g_tfs)], lr=1e-2)
+    n_all = float(sum(w * h for w, h in sizes) * 4)
+
+    if args.pmc_probe:   # 3 batched runs and one 256 MiB device copy (counter calibration)
+        for _ in range(3):
+            plan.run()
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": src.numel() * 4,
+                          "samples_per_step": float(sum(v.termination.sum().item() for v in views))}))
+        return None
+
+    # accuracy sample: the first view of each volume until 2.5e7 GPU samples (also the CPU baseline's)
     plan.run()
     torch.cuda.synchronize()
+    first = {}
+    for i in range(len(views)):
+        first.setdefault(vidx[i], i)
+    sample_idx, acc_n = [], 0.0
+    for k in sorted(first):
+        if acc_n >= 2.5e7 and len(sample_idx) >= 3:
+            break
+        i = first[k]
+        sample_idx.append(i)
+        acc_n += float(views[i].termination.sum().item())
+    accuracy, cpu_sample = None, None
+    if not args.profile:
+        accuracy, cpu_sample = c5_accuracy(bt, wl, vols, tfs, vps, sizes, vidx, steps_w, targets, sample_idx, n_all)
+
+    # plain leg: per view the reference-structured forward, torch's L2 gradient and the inversion
+    # adjoint (density + TF gradients accumulated into the same tensors), then the same Adam step
+    adam = bt.Adam([(v, g, None, (0.0, 1.0)) for v, g in zip(vols, g_vols)] +
+                   [(t, g, 4, (0.0, 1e3)) for t, g in zip(tfs, g_tfs)], lr=1e-2)
+    plain = None
+    if not args.profile:
+        per = []
+        for i in range(len(views)):
+            w, h = sizes[i]
+            inp = pr.RendererInputs()
+            inp.screen_size = pr.int2(w, h)
+            inp.volume = vols[vidx[i]].unsqueeze(0)
+            inp.volume_filter_mode = pr.VolumeFilterMode.Trilinear
+            inp.box_min = pr.real3(-0.5, -0.5, -0.5)
+            inp.box_size = pr.real3(1.0, 1.0, 1.0)
+            inp.camera_mode = pr.CameraMode.ReferenceFrame
+            inp.camera = pr.CameraReferenceFrame(vps[i].unsqueeze(0), wl.fov_y)
+            inp.step_size = steps_w[i]
+            inp.tf_mode = pr.TFMode.Linear
+            inp.tf = tfs[vidx[i]].unsqueeze(0)
+            inp.blend_mode = pr.BlendMode.BeerLambert
+            a = pr.AdjointOutputs()
+            a.has_volume_derivatives = True
+            a.adj_volume = g_vols[vidx[i]].unsqueeze(0)
+            a.has_tf_derivatives = True
+            a.tf_delayed_accumulation = True   # diffdvr.Renderer default (pytests/diffdvr/renderer.py:270)
+            a.adj_tf = g_tfs[vidx[i]].unsqueeze(0)
+            cb = torch.empty(1, h, w, 4, device=dev)
+            tb = torch.empty(1, h, w, dtype=torch.int32, device=dev)
+            per.append((inp, a, cb, tb, targets[i].unsqueeze(0)))
+
+        def plain_grads():
+            for g in g_vols + g_tfs:
+                g.zero_()
+            for inp, a, cb, tb, tg in per:
+                o = pr.RendererOutputs(cb, tb)
+                pr.Renderer.render_forward(inp, o, path="plain")
+                pr.Renderer.render_adjoint(inp, o, 2.0 * (cb - tg) / n_all, a, method="inversion", path="plain")
+
+        # gradients of both forms at the same parameters
+        plan.run()
+        torch.cuda.synchronize()
+        bg = [g.clone() for g in g_vols[:1] + g_tfs]
+        plain_grads()
+        torch.cuda.synchronize()
+        pg = [g.clone() for g in g_vols[:1] + g_tfs]
+        d_vol = float(torch.linalg.norm(pg[0] - bg[0]) / torch.linalg.norm(bg[0]))
+        d_tf = float(torch.linalg.norm(torch.cat([(a - b).flatten() for a, b in zip(pg[1:], bg[1:])])) /
+                     torch.linalg.norm(torch.cat([b.flatten() for b in bg[1:]])))
+        del bg, pg
+
+        def plain_step():
+            plain_grads()
+            adam.step()
+
+        plain_step()
+        ms_plain = timed(plain_step, 2, stream)
+        plain = {"ms_per_step": ms_plain, "density_grad_rel_diff_vs_batched": d_vol,
+                 "tf_grad_rel_diff_vs_batched": d_tf}
+
+    for _ in range(args.warmup):
+        plan.run()
+        adam.step()
+    torch.cuda.synchronize()
+    n0 = float(sum(v.termination.sum().item() for v in views))
     loss0 = float(loss.item())
     graph = bt.GraphedSteps(plan, adam, G)
     torch.cuda.synchronize()
     ms_graph = timed(graph.replay, reps, stream)
-    n_samples = float(sum(v.termination.sum().item() for v in views))  # last step's sample count
+    n1 = float(sum(v.termination.sum().item() for v in views))   # the last step's sample count
+    n_samples = 0.5 * (n0 + n1)
     ms = ms_graph / G
     value = n_samples / (ms * 1e-3)
-    k_ms = float(np.mean([timed(lambda: plan.run(True), 1, stream) for _ in range(2)]))
+    loss1 = float(loss.item())
+    # the persistent kernel alone (HIP events around it on the launch stream), outside the graph
+    k_ms = float(np.median([plan.kernel_ms() for _ in range(5)]))
     config = {"workload": "C5: 16 blob volumes {64..256}^3, 4-32 cameras each at 128..512, linear TF per volume, "
                           "step 0.5 voxel, L2 vs second volumes, density + TF gradient, HIP Adam, "
                           f"{G} steps captured as one HIP graph",
               "volumes": len(vols), "views": len(views), "rays": int(sum(w * h for w, h in sizes)),
-              "samples_per_step": n_samples, "graph_steps": G, "loss_first": loss0,
-              "loss_last": float(loss.item()), "adam_steps": adam.step_count, **plan.options}
-    return result_line(args, "c5", value, ms, n_samples, config, k_ms, {"autotuned": tuned is not None})
+              "samples_per_step": n_samples, "samples_first_last": [n0, n1], "graph_steps": G, "graph_replays": reps,
+              "loss_before_graph": loss0, "loss_last_step": loss1, "adam_steps": adam.step_count, **plan.options}
+    result = result_line(args, "c5", value, ms, n_samples, config, k_ms, {"autotuned": tuned is not None})
+    if plain is not None:
+        plain["value"] = n_samples / (plain["ms_per_step"] * 1e-3)
+        plain["speedup_batched_vs_plain"] = plain["ms_per_step"] / ms
+        result["plain"] = plain
+    if accuracy is not None:
+        result["accuracy"] = accuracy
+        if not args.no_cpu_baseline:
+            result["cpu_baseline"] = cpu_baseline_c5(cpu_sample, n_all)
+    return result
+
+
+def c5_accuracy(bt, wl, vols, tfs, vps, sizes, vidx, steps_w, targets, idx, n_all):
+    """FP64 oracle vs the batched kernel on the sampled views: a plan of those views alone (same
+    kernel instance) gives their gradients; colours at the GPU's sample counts, density (interior
+    voxels) and TF gradients of the exact-replay adjoint.  The headline gradient errors leave out
+    the rays with a sample within FP32 rounding of a control point (the oracle's kink census),
+    whose loss gradient is zeroed on both sides; the all-ray errors and the FP32 CPU twin's
+    all-ray density error are given beside them."""
+    import numpy as np
+    import torch
+    from oracle import oracle
+
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    used = sorted({vidx[i] for i in idx})
+    g_v = {k: torch.zeros_like(vols[k]) for k in used}
+    g_t = {k: torch.zeros_like(tfs[k]) for k in used}
+    tg_sub = [targets[i].clone() for i in idx]
+    sub = [bt.BatchView(volume=used.index(vidx[i]), tf=used.index(vidx[i]), width=sizes[i][0], height=sizes[i][1],
+                        camera=vps[i], fov_y=wl.fov_y, step=steps_w[i], target=tg_sub[j], loss_scale=1.0 / n_all)
+           for j, i in enumerate(idx)]
+    p = bt.BatchPlan([bt.BatchVolume(vols[k], grad=g_v[k]) for k in used], [bt.BatchTf(tfs[k], g_t[k]) for k in used],
+                     sub, volume_filter_mode=1, tf_mode=2, blend_mode=0, loss="l2")
+    p.run()
+    torch.cuda.synchronize()
+    got_all = {k: (g_v[k].double().cpu().numpy(), g_t[k].double().cpu().numpy()) for k in used}
+    ferr, sample, kink_samples, samples = 0.0, [], 0, 0
+    want = {k: {n: 0.0 for n in ("v", "t", "v_clean", "t_clean", "v_twin")} for k in used}
+    for j, i in enumerate(idx):
+        k = vidx[i]
+        w, h = sizes[i]
+        vol = vols[k].double().cpu().numpy()
+        tf = tfs[k].double().cpu().numpy()
+        vp = vps[i].double().cpu().numpy()
+        tg = targets[i].double().cpu().numpy()
+        s = c5_oracle_scene(vol, tf, vp, w, h, steps_w[i], wl.fov_y)
+        t = sub[j].termination.cpu().numpy()[None]
+        c64, _ = oracle.forward(s, frozen_term=t)
+        c = sub[j].color.double().cpu().numpy()[None]
+        ferr = max(ferr, float(np.abs(c - c64).max() / np.abs(c64).max()))
+        adj = 2.0 * (c - tg[None]) / n_all
+        g = oracle.adjoint(s, c64, t, adj, grads=("volume", "tf"), method=oracle.EXACT_REPLAY)
+        want[k]["v"] = want[k]["v"] + g["volume"][0]
+        want[k]["t"] = want[k]["t"] + g["tf"][0]
+        c32, _ = oracle.forward(s, dtype=np.float32, frozen_term=t)
+        want[k]["v_twin"] = want[k]["v_twin"] + oracle.adjoint(s, c32, t, adj, grads=("volume",), dtype=np.float32,
+                                                                method=oracle.EXACT_REPLAY)["volume"][0]
+        census = oracle.kink_census(s, t)[0]
+        kink_samples += int(census.sum())
+        samples += int(t.sum())
+        kinked = census > 0
+        kt = torch.as_tensor(kinked, device=tg_sub[j].device)
+        tg_sub[j][kt] = sub[j].color[kt]
+        adj[0][kinked] = 0.0
+        g = oracle.adjoint(s, c64, t, adj, grads=("volume", "tf"), method=oracle.EXACT_REPLAY)
+        want[k]["v_clean"] = want[k]["v_clean"] + g["volume"][0]
+        want[k]["t_clean"] = want[k]["t_clean"] + g["tf"][0]
+        sample.append((vol, tf, vp, w, h, steps_w[i], wl.fov_y, tg))
+    p.run()
+    torch.cuda.synchronize()
+    inner = (slice(2, -2),) * 3
+
+    def err(pairs):
+        num = sum(float(np.sum((a - b) ** 2)) for a, b in pairs)
+        den = sum(float(np.sum(b ** 2)) for _, b in pairs)
+        return math.sqrt(num / den)
+
+    acc = {"views": [int(i) for i in idx], "forward_rel_err_vs_fp64": ferr,
+           "density_grad_rel_err_vs_fp64": err([(g_v[k].double().cpu().numpy()[inner], want[k]["v_clean"][inner])
+                                                for k in used]),
+           "tf_grad_rel_err_vs_fp64": err([(g_t[k].double().cpu().numpy(), want[k]["t_clean"]) for k in used]),
+           "density_grad_rel_err_all_rays": err([(got_all[k][0][inner], want[k]["v"][inner]) for k in used]),
+           "tf_grad_rel_err_all_rays": err([(got_all[k][1], want[k]["t"]) for k in used]),
+           "fp32_cpu_twin_density_grad_rel_err_all_rays": err([(want[k]["v_twin"][inner], want[k]["v"][inner])
+                                                               for k in used]),
+           "kink_samples": kink_samples, "samples": samples,
+           "note": "density gradients on interior voxels; headline gradient errors leave out the rays with a "
+                   "sample within FP32 rounding of a control point (loss gradient zeroed on both sides)"}
+    return acc, sample
 
 
 # ---------------------------------------------------------------------------

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -466,6 +467,42 @@ dvr::CkptParams make_ckpt(const Checked& c, int interval)
 }
 
 
+// Host-side stamps of the checkpoint buffers written by dvr_render_forward_ex: the adjoint that
+// reads one must use the same checkpoint interval and image batch, and may read densities only
+// from the buffer the same forward wrote (else it would read another layout, or uninitialised
+// memory, as valid data).  Keyed by the checkpoint buffer's address; a later forward into the
+// same buffer replaces the stamp.
+struct CkptStamp {
+    int K, B, H, W;
+    const void* densities;  // written by that forward, or NULL
+};
+std::mutex g_stampMu;
+std::map<const void*, CkptStamp> g_stamps;
+constexpr size_t kMaxStamps = 4096;
+
+void stamp_checkpoints(const void* ck, const CkptStamp& st)
+{
+    std::lock_guard<std::mutex> lk(g_stampMu);
+    if (g_stamps.size() >= kMaxStamps && !g_stamps.count(ck)) g_stamps.erase(g_stamps.begin());
+    g_stamps[ck] = st;
+}
+
+void check_stamp(const void* ck, int K, const Checked& c, const void* densities)
+{
+    std::lock_guard<std::mutex> lk(g_stampMu);
+    auto it = g_stamps.find(ck);
+    if (it == g_stamps.end())
+        fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints were not written by dvr_render_forward_ex");
+    const CkptStamp& st = it->second;
+    if (st.K != K)
+        fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints were written with checkpoint_interval %d, read with %d", st.K, K);
+    if (st.B != c.B || st.H != c.H || st.W != c.W)
+        fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints were written for a %dx%dx%d image batch, read for %dx%dx%d",
+             st.B, st.H, st.W, c.B, c.H, c.W);
+    if (densities && densities != st.densities)
+        fail(DVR_ERR_INVALID_ARGUMENT, "densities were not written by the forward that wrote these checkpoints");
+}
+
 void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forward_options* opt, void* stream)
 {
     Checked c = check_inputs(inputs);
@@ -510,6 +547,7 @@ void do_forward(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_forwar
             }
         }
         check_hip(dvr::launch_forward_tiled(c.P, CK, c.filter, c.tfm, c.blend, s), "forward (tiled) launch");
+        if (K > 0) stamp_checkpoints(CK.buf, CkptStamp{K, c.B, c.H, c.W, CK.ds});
     }
 }
 
@@ -548,6 +586,9 @@ void do_adjoint(const dvr_inputs* inputs, const dvr_outputs* fwd, const dvr_tens
         for (int d = 0; d < ck.ndim; ++d) numel *= ck.sizes[d];
         if (ck.dtype != DVR_FLOAT32 || ck.device != c.device || numel < need)
             fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints do not match these inputs and checkpoint_interval");
+        if (ck.ndim > 0 && ck.strides[ck.ndim - 1] != 1 && numel > 1)
+            fail(DVR_ERR_INVALID_ARGUMENT, "checkpoints must be contiguous");
+        check_stamp(ck.data, K, c, given(opt->densities) ? opt->densities.data : nullptr);
         CK.buf = (float*)ck.data;
         fromFwd = true;
         if (given(opt->densities)) {  // written by the forward launch that wrote the checkpoints

@@ -133,7 +133,10 @@ typedef struct dvr_adjoint_options {
     int32_t path;                /* DVR_PATH_* */
     int32_t method;              /* DVR_ADJOINT_* */
     int32_t checkpoint_interval; /* K in {8,16,32,64}; 0 = default (16) */
-    dvr_tensor checkpoints;      /* optional: written by dvr_render_forward_ex with the same K */
+    dvr_tensor checkpoints;      /* optional: written by dvr_render_forward_ex with the same K and image batch
+                                    (checked: the library keeps a host-side stamp of every checkpoint buffer a
+                                    forward writes -- K, B x H x W and the density buffer it filled -- and
+                                    rejects a mismatch with DVR_ERR_INVALID_ARGUMENT) */
     dvr_tensor densities;        /* optional, with checkpoints: the densities that same forward call wrote; the
                                     reverse march reads them instead of re-interpolating the volume (without
                                     checkpoints the library uses a scratch store of its own when it fits 2 GiB) */

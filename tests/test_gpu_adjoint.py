@@ -277,3 +277,38 @@ def test_generate_rays_and_adjoint():
     ((rs * t32(a_s)).sum() + (rd * t32(a_d)).sum()).backward()
     want = oracle.generate_rays_adjoint(vp.double().numpy(), H, W, fov, a_s, a_d)
     assert norm_err(vpd.grad.double().cpu().numpy(), want) < 1e-5
+
+
+def test_checkpoint_handover_is_checked():
+    """dvr_render_adjoint_ex reads only checkpoints (and densities) that dvr_render_forward_ex wrote
+    for the same checkpoint interval and image batch."""
+    rng = np.random.default_rng(31)
+    s = camera_scene(rng, 16, 12, tf_mode=1, camera=2)
+    inp = gpu_inputs(s)
+    B, H, W = 1, 12, 16
+    color = torch.empty(B, H, W, 4, device=dev())
+    term = torch.empty(B, H, W, dtype=torch.int32, device=dev())
+    ck = torch.empty(pr.Renderer.checkpoint_buffer_elements(inp, 16), device=dev())
+    ds = torch.empty(pr.Renderer.density_buffer_elements(inp), device=dev())
+    pr.Renderer.render_forward(inp, pr.RendererOutputs(color, term), path="tiled", checkpoint_interval=16,
+                               checkpoints=ck)
+    adj = pr.AdjointOutputs()
+    adj.has_tf_derivatives = True
+    adj.adj_tf = torch.zeros_like(inp.tf)
+    g = torch.ones(B, H, W, 4, device=dev())
+    outs = pr.RendererOutputs(color, term)
+    with pytest.raises(RuntimeError, match="written with checkpoint_interval 16, read with 32"):
+        pr.Renderer.render_adjoint(inp, outs, g, adj, method="checkpoint", checkpoint_interval=32, checkpoints=ck)
+    with pytest.raises(RuntimeError, match="densities were not written"):
+        pr.Renderer.render_adjoint(inp, outs, g, adj, method="checkpoint", checkpoint_interval=16, checkpoints=ck,
+                                   densities=ds)
+    other = torch.empty_like(ck)
+    with pytest.raises(RuntimeError, match="not written by dvr_render_forward_ex"):
+        pr.Renderer.render_adjoint(inp, outs, g, adj, method="checkpoint", checkpoint_interval=16, checkpoints=other)
+    # the matching handover works and equals the adjoint that replays the forward itself
+    pr.Renderer.render_adjoint(inp, outs, g, adj, method="checkpoint", checkpoint_interval=16, checkpoints=ck)
+    a1 = adj.adj_tf.clone()
+    adj.adj_tf.zero_()
+    pr.Renderer.render_adjoint(inp, outs, g, adj, method="checkpoint", checkpoint_interval=16)
+    torch.cuda.synchronize()
+    assert float((a1 - adj.adj_tf).abs().max()) <= 1e-5 * float(a1.abs().max())

@@ -4,7 +4,9 @@ Protocol: the termination index must agree with the free-running FP64 oracle
 for >= 99.9% of the rays (smooth workloads; >= 99% on white-noise volumes)
 and by at most one sample everywhere; colours are compared with the oracle
 replaying the GPU's sample count (FP64), error = max|gpu - oracle| / max|oracle|:
-1e-5 on the C1 workload, 1e-4 on white-noise stress scenes (see
+1e-5 on the C1 workload and on smooth volumes in every camera / filter / TF /
+blend mode (tiled path; 4e-5 for the plain path's FP32 formulation), 1e-4 on
+white-noise stress scenes (see
 tests/test_oracle_render.py::test_fp32_twin_matches_fp64_forward).
 """
 import numpy as np
@@ -58,6 +60,40 @@ def test_mode_matrix(filter, tf_mode, blend, camera):
     for path in ("tiled", "plain"):
         _, color, term = render(s, path=path)
         check_forward(s, color, term, 1e-4, 0.99)
+
+
+@pytest.mark.parametrize("filter", [0, 1])
+@pytest.mark.parametrize("tf_mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("blend", [0, 1])
+@pytest.mark.parametrize("camera", [0, 1, 2])
+def test_mode_matrix_smooth(filter, tf_mode, blend, camera):
+    """Every camera / filter / TF / blend mode on a smooth (blob) volume: the tiled path within
+    1e-5 of the FP64 oracle, the plain path (the reference's FP32 formulation) within 4e-5.
+    Nearest filtering jumps in value where a sample rounds to the other voxel: rays with a
+    sample within FP32 rounding of such an edge (the oracle's kink census) are left out of the
+    bound, and at most 0.1% of the samples may be such samples.  At most one ray in a thousand
+    (and at least one) may end one sample earlier or later than the free-running oracle."""
+    rng = np.random.default_rng(5000 + 100 * filter + 10 * tf_mode + 3 * blend + camera)
+    s = camera_scene(rng, 23, 19, B=2, filter=filter, tf_mode=tf_mode, blend=blend, camera=camera, smooth=True,
+                     X=20, Y=18, Z=22, step=0.02)
+    for path, tol in (("tiled", 1e-5), ("plain", 4e-5)):
+        _, color, term = render(s, path=path)
+        t_gpu = term.cpu().numpy()
+        c_free, t_free = oracle.forward(s)
+        c_ora, _ = oracle.forward(s, frozen_term=t_gpu)
+        # a ray may stop one sample apart where its transmittance ties with the early-out threshold
+        # within rounding (1e-4 relative); any other disagreement is limited to 0.1% (at least 1) ray
+        thr = 1e-5
+        tie = (np.abs((1 - c_free[..., 3]) - thr) <= 1e-4 * thr) | (np.abs((1 - c_ora[..., 3]) - thr) <= 1e-4 * thr)
+        off = t_gpu != t_free
+        assert np.abs(t_gpu - t_free).max() <= 1 and np.sum(off & ~tie) <= max(1, 1e-3 * t_gpu.size), \
+            (int(off.sum()), int((off & ~tie).sum()))
+        census = oracle.kink_census(s, t_gpu) if filter == 0 else np.zeros_like(t_gpu)
+        clean = census == 0
+        assert census.sum() <= 1e-3 * t_gpu.sum(), (int(census.sum()), int(t_gpu.sum()))
+        err = np.abs(color.double().cpu().numpy() - c_ora)[clean].max() / np.abs(c_ora).max()
+        print(f"{path}: smooth forward err {err:.2e}, rays left out {int((~clean).sum())}")
+        assert err <= tol, (path, err)
 
 
 def test_strided_and_broadcast_inputs():
