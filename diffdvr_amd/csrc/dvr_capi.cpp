@@ -610,8 +610,10 @@ void do_adjoint(const dvr_inputs* inputs, const dvr_outputs* fwd, const dvr_tens
               "adjoint (tiled) launch");
 }
 
+// fixed: the allocation-free variant's scratch (a getter of the device's static array and its
+// size in floats); NULL = the library's grow-only workspace
 void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss, dvr_adjoint_outputs* ao,
-              int K, void* stream)
+              int K, void* stream, const dvr_host::FixedScratch* fixed = nullptr)
 {
     Checked c = check_inputs(inputs);
     no_preshaded(c, "fused rendering");
@@ -645,10 +647,21 @@ void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* lo
     DeviceGuard g(c.device);
     hipStream_t s = (hipStream_t)stream;
     const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * CK.cap * dvr::kBlock;
-    CK.buf = (float*)workspace(c.device, size_t(need) * sizeof(float), s);
-    if (double(density_elements(c)) * sizeof(float) <= kMaxDensityStore) {
-        CK.ds = (float*)workspace(c.device, size_t(density_elements(c)) * sizeof(float), s, 1);
-        CK.dsCap = density_cap(c);
+    if (fixed) {
+        // checkpoints in the device's static scratch; no density store (the reverse march
+        // re-interpolates: the same densities)
+        if (need > fixed->floats)
+            fail(DVR_ERR_INVALID_ARGUMENT,
+                 "dvr_render_fused_noalloc needs %lld floats of checkpoint scratch for these inputs and "
+                 "checkpoint_interval %d, its static scratch holds %lld (use a larger interval or fewer pixels "
+                 "per call)", (long long)need, K, (long long)fixed->floats);
+        CK.buf = fixed->get(c.device);
+    } else {
+        CK.buf = (float*)workspace(c.device, size_t(need) * sizeof(float), s);
+        if (double(density_elements(c)) * sizeof(float) <= kMaxDensityStore) {
+            CK.ds = (float*)workspace(c.device, size_t(density_elements(c)) * sizeof(float), s, 1);
+            CK.dsCap = density_cap(c);
+        }
     }
     check_hip(dvr::launch_adjoint_tiled(c.P, A, CK, L, true, false, c.filter, c.tfm, c.blend, s), "fused launch");
 }
@@ -1122,3 +1135,12 @@ dvr_status dvr_timer_elapsed_ms(dvr_timer* t, float* ms)
 }
 
 }  // extern "C"
+
+namespace dvr_host {
+dvr_status fused_fixed_scratch(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss,
+                               dvr_adjoint_outputs* adj_outputs, int32_t checkpoint_interval, void* stream,
+                               const FixedScratch& fixed)
+{
+    return guarded([&] { do_fused(inputs, outputs, loss, adj_outputs, checkpoint_interval, stream, &fixed); });
+}
+}  // namespace dvr_host

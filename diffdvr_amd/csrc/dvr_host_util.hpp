@@ -100,4 +100,15 @@ struct DeviceGuard {
     }
 };
 
+// Scratch of the allocation-free fused entry point: `get` returns the calling thread's current
+// device's static array (it may fail(); it is called with that device current), `floats` its size.
+struct FixedScratch {
+    float* (*get)(int device);
+    int64_t floats;
+};
+// dvr_render_fused with the checkpoints in that scratch (dvr_capi.cpp)
+dvr_status fused_fixed_scratch(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss,
+                               dvr_adjoint_outputs* adj_outputs, int32_t checkpoint_interval, void* stream,
+                               const FixedScratch& fixed);
+
 }  // namespace dvr_host

@@ -385,6 +385,7 @@ struct AdamHandle {
 PYBIND11_MODULE(_pyrenderer, m)
 {
     m.attr("MAX_TILED_TF_ENTRIES") = DVR_MAX_TILED_TF_ENTRIES;
+    m.attr("NOALLOC_SCRATCH_FLOATS") = (int64_t)DVR_NOALLOC_SCRATCH_FLOATS;
     m.doc() = "MI355X (gfx950) differentiable direct volume renderer: pyrenderer API over libdvr_hip";
     m.def("use_double_precision", []() { return dvr_use_double_precision(); },
           "Returns 0: this library computes in single precision.");
@@ -578,6 +579,25 @@ PYBIND11_MODULE(_pyrenderer, m)
               l.loss = to_tensor(loss_out);
               dvr_adjoint_outputs ao = pack_adjoint(adj);
               check(dvr_render_fused(&p.in, &o, &l, &ao, checkpoint_interval, current_stream(p.in.volume.device)));
+          },
+          py::arg("inputs"), py::arg("outputs"), py::arg("loss"), py::arg("target"), py::arg("adj_outputs"),
+          py::arg("loss_out") = py::none(), py::arg("scale") = 0.0f, py::arg("checkpoint_interval") = 0);
+    m.def("render_fused_noalloc",
+          [](const RendererInputs& inputs, RendererOutputs& outputs, const std::string& loss, py::object target,
+             AdjointOutputs& adj, py::object loss_out, float scale, int checkpoint_interval) {
+              PackedInputs p = pack_inputs(inputs);
+              dvr_outputs o = pack_outputs(outputs);
+              dvr_loss l;
+              std::memset(&l, 0, sizeof(l));
+              if (loss == "l1") l.kind = DVR_LOSS_L1;
+              else if (loss == "l2" || loss == "mse") l.kind = DVR_LOSS_L2;
+              else throw std::invalid_argument("loss must be 'l1' or 'l2'");
+              l.target = to_tensor(target);
+              l.scale = scale;
+              l.loss = to_tensor(loss_out);
+              dvr_adjoint_outputs ao = pack_adjoint(adj);
+              check(dvr_render_fused_noalloc(&p.in, &o, &l, &ao, checkpoint_interval,
+                                             current_stream(p.in.volume.device)));
           },
           py::arg("inputs"), py::arg("outputs"), py::arg("loss"), py::arg("target"), py::arg("adj_outputs"),
           py::arg("loss_out") = py::none(), py::arg("scale") = 0.0f, py::arg("checkpoint_interval") = 0);

@@ -203,6 +203,20 @@ typedef struct dvr_loss {
 dvr_status dvr_render_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss,
                             dvr_adjoint_outputs* adj_outputs, int32_t checkpoint_interval, void* stream);
 
+/* dvr_render_fused without any allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as dvr_render_fused (the reverse march
+ * re-interpolates instead of reading a density store: the same densities), with the checkpoints in
+ * a static per-device array of DVR_NOALLOC_SCRATCH_FLOATS floats (32 MiB, held by every GPU the
+ * library runs on until the process ends).  Inputs whose checkpoints need more
+ * (dvr_checkpoint_buffer_elements) return DVR_ERR_INVALID_ARGUMENT: use a larger
+ * checkpoint_interval or fewer pixels per call.  The first call on a device sets the array up
+ * (hipGetSymbolAddress, which makes the runtime load it): make that call before capturing a graph.
+ * Calls that overlap on one GPU (other streams) share the array and race: serialise them. */
+#define DVR_NOALLOC_SCRATCH_FLOATS (8 * 1024 * 1024)
+dvr_status dvr_render_fused_noalloc(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss,
+                                    dvr_adjoint_outputs* adj_outputs, int32_t checkpoint_interval, void* stream);
+
 /* Number of float32 elements a checkpoint buffer needs for these inputs
  * and interval K (0 if K == 0). */
 int64_t dvr_checkpoint_buffer_elements(const dvr_inputs* inputs, int32_t checkpoint_interval);
