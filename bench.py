@@ -15,6 +15,9 @@ Workloads (synthetic, seeded, identical on every host; diffdvr_amd/scenes.py):
      per-view form (reference-structured forward + inversion adjoint per view,
      then the same Adam step), the FP32 CPU port on sampled views, and the FP64
      oracle's error on those views
+  c1: forward only, the CPU-runnable configuration: 64^3 blobs, 5-point linear TF,
+     one 128^2 view at lat 30 / lon 45, step 0.5 voxel; the FP32 CPU port on the
+     whole image beside it (1 warm-up, median of 5)
   c2: 128^3 blobs, 64x4 texture TF,
      8 views 256^2 at latitude 20, step 0.25 voxel, L1 vs a second TF, TF gradient
   c3: 256^3 blobs + noise, identity TF x10, 64 Fibonacci views 256^2, step 0.5
@@ -27,7 +30,7 @@ Workloads (synthetic, seeded, identical on every host; diffdvr_amd/scenes.py):
      adjoint, with best fit, log-MSE and importance sampling timed beside it
 
 Prints ONE JSON line.  Options:
-  --workload c2|c3|c4|c5|fwd|preshade  --steps K --warmup W  --gpus N (only 1)
+  --workload c1|c2|c3|c4|c5|fwd|preshade  --steps K --warmup W  --gpus N (only 1)
   --no-cpu-baseline  --profile (short run for rocprofv3: no plain leg, no baseline)
   --path persistent|fused (c2: persistent plan, or the single-batch fused kernel)
   --checkpoint-interval K  --wave-width W  --grid G  (all 0 = autotuned; c4 uses defaults)
@@ -50,14 +53,14 @@ HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
 # gathers in the forward march, 32 B again for the adjoint, plus 8 atomic
 # read-modify-writes of 8 B for a volume gradient (TF and camera gradients are
 # privatised in LDS / registers: 0 B per sample)
-BYTES_PER_SAMPLE = {"c2": 64, "c3": 128, "c4": 64, "c5": 128, "fwd": 32}
+BYTES_PER_SAMPLE = {"c1": 32, "c2": 64, "c3": 128, "c4": 64, "c5": 128, "fwd": 32}
 METRIC = "fwd+adjoint ray-samples/s on 1 MI355X, batched vs plain, at 1e-4 rel grad err"
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="c5", choices=["c2", "c3", "c4", "c5", "fwd", "preshade"])
+    p.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c4", "c5", "fwd", "preshade"])
     p.add_argument("--steps", type=int, default=0,
                    help="timed steps (0 = per-workload default; c5: rounded up to whole 100-step graphs)")
     p.add_argument("--warmup", type=int, default=3)
@@ -191,6 +194,82 @@ def cpu_baseline_c2(wl, target, color, term, adj_tf, scale):
                   f"FP32 forward + L1 gradient + inversion adjoint with TF gradient, the OpenMP restatement of "
                   f"renderer_cpu.cpp (schedule(guided) over pixels, omp atomic scatters)",
     }, ferr, gerr
+
+
+# ---------------------------------------------------------------------------
+# C1: forward only (the CPU-runnable configuration)
+# ---------------------------------------------------------------------------
+def bench_c1(args):
+    import numpy as np
+    import torch
+    from diffdvr_amd import pyrenderer as pr
+    from diffdvr_amd import scenes
+    from oracle import oracle
+
+    dev = torch.device("cuda:0")
+    wl = scenes.workload_c1().to(dev)
+    H, W = wl.height, wl.width
+    steps = args.steps or 200
+    args.steps = steps
+    stream = torch.cuda.current_stream()
+    inp = pr.RendererInputs()
+    inp.screen_size = pr.int2(W, H)
+    inp.volume = wl.volume
+    inp.volume_filter_mode = pr.VolumeFilterMode.Trilinear
+    inp.box_min = pr.real3(-0.5, -0.5, -0.5)
+    inp.box_size = pr.real3(1.0, 1.0, 1.0)
+    inp.camera_mode = pr.CameraMode.ReferenceFrame
+    inp.camera = pr.CameraReferenceFrame(wl.viewports, wl.fov_y)
+    inp.step_size = wl.step
+    inp.tf_mode = pr.TFMode.Linear
+    inp.tf = wl.tf
+    inp.blend_mode = pr.BlendMode.BeerLambert
+    color = torch.empty(1, H, W, 4, device=dev)
+    term = torch.empty(1, H, W, dtype=torch.int32, device=dev)
+    outs = pr.RendererOutputs(color, term)
+
+    def tiled():
+        pr.Renderer.render_forward(inp, outs, path="tiled")
+
+    def plain():
+        pr.Renderer.render_forward(inp, outs, path="plain")
+
+    for _ in range(args.warmup):
+        tiled()
+        plain()
+    torch.cuda.synchronize()
+    ms_plain = timed(plain, steps, stream)
+    ms = timed(tiled, steps, stream)
+    n = float(term.sum().item())
+    config = {"workload": "C1: 64^3 blob volume (8 blobs), 5-point linear TF, one 128x128 view (lat 30, lon 45, "
+                          "radius 2.5), step 0.5 voxel, forward only", "image": [H, W], "samples_per_step": n,
+              "path": "tiled"}
+    result = result_line(args, "c1", n / (ms * 1e-3), ms, n, config, ms, kernel="dvr::k_forward_tiled",
+                         metric="forward ray-samples/s on 1 MI355X")
+    result["plain"] = {"value": n / (ms_plain * 1e-3), "ms_per_step": ms_plain, "speedup_tiled_vs_plain": ms_plain / ms}
+    s = dict(W=W, H=H, filter=1, camera=2, tf_mode=2, blend=0, early_out=1 - 1e-5, fov_y=wl.fov_y,
+             volume=wl.volume.double().cpu().numpy(), box_min=np.full((1, 3), -0.5), box_size=np.ones((1, 3)),
+             step=np.full((1, 1, 1), wl.step), tf=wl.tf.double().cpu().numpy(),
+             cam_matrix=wl.viewports.double().cpu().numpy())
+    c64, _ = oracle.forward(s, frozen_term=term.cpu().numpy())
+    result["accuracy"] = {"forward_rel_err_vs_fp64": float(np.abs(color.double().cpu().numpy() - c64).max() /
+                                                           np.abs(c64).max())}
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        oracle.forward(s, dtype=np.float32)
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            _, t32 = oracle.forward(s, dtype=np.float32)
+            times.append(time.perf_counter() - t0)
+        med = float(np.median(times))
+        result["cpu_baseline"] = {
+            "value": float(t32.sum()) / med, "unit": "ray-samples/s", "cores": threads, "kind": "port",
+            "sample": f"the whole workload ({W}x{H}, {float(t32.sum()):.3g} samples): FP32 forward, the OpenMP "
+                      f"restatement of renderer_cpu.cpp; 1 warm-up, median of 5 ({med * 1e3:.2f} ms)",
+            **host_cpu()}
+    return result
 
 
 def bench_c2(args):
@@ -1070,7 +1149,7 @@ def main():
         print(f"bench.py: only --gpus 1 is supported; this build measures one MI355X "
               f"(multi-GPU sharding is a design note, not a build target)", file=sys.stderr)
         sys.exit(2)
-    fn = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "fwd": bench_fwd,
+    fn = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "fwd": bench_fwd,
           "preshade": bench_preshade}[args.workload]
     result = fn(args)
     if result is not None:

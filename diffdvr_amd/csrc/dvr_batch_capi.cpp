@@ -5,8 +5,12 @@
 // optimisation steps can be captured into HIP graphs.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -89,9 +93,16 @@ struct dvr_batch_plan {
     dvr::BVol* dVols = nullptr;
     dvr::BTf* dTfs = nullptr;
     unsigned* dQueues = nullptr;
-    dvr::ZeroSeg* dSegsAll = nullptr;   // [queues, gradients...]
+    dvr::ZeroSeg* dSegsAll = nullptr;   // [queues, bricked scratch..., gradients...]
     int nSegsAll = 0;
     long long maxSegAll = 0;
+    int nSegsAlways = 1;                // cleared by every run: queues and bricked scratch
+    long long maxSegAlways = dvr::kNumXcd;
+    float* dBricks = nullptr;           // bricked volume-gradient scratch of every gradient volume
+    size_t brickFloats = 0;
+    dvr::DebrickSeg* dDebrick = nullptr;
+    int nDebrick = 0;
+    long long maxBricks = 0;
     float* dCk = nullptr;
     size_t ckFloats = 0;
     float* dDs = nullptr;   // density store of the forward march (adjoint plans), or NULL
@@ -103,7 +114,7 @@ struct dvr_batch_plan {
     ~dvr_batch_plan()
     {
         dev_free(dViews); dev_free(dVols); dev_free(dTfs); dev_free(dQueues); dev_free(dSegsAll); dev_free(dCk);
-        dev_free(dDs);
+        dev_free(dDs); dev_free(dBricks); dev_free(dDebrick);
     }
 
     // floats between the TF-gradient table copies (odd: the copies start in different LDS banks);
@@ -204,8 +215,8 @@ struct dvr_batch_plan {
 
     void zero(bool gradients, hipStream_t s) const
     {
-        const int n = gradients ? nSegsAll : 1;
-        const long long mx = gradients ? maxSegAll : dvr::kNumXcd;
+        const int n = gradients ? nSegsAll : nSegsAlways;
+        const long long mx = gradients ? maxSegAll : maxSegAlways;
         check_hip(dvr::launch_zero_segments(dSegsAll, n, mx, s), "zero launch");
     }
 
@@ -218,6 +229,7 @@ struct dvr_batch_plan {
             check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, grid, lds, s, nullptr),
                       "batched launch");
         if (evEnd) check_hip(hipEventRecord(evEnd, s), "hipEventRecord");
+        if (nDebrick > 0) check_hip(dvr::launch_debrick(dDebrick, nDebrick, maxBricks, s), "debrick launch");
     }
 };
 
@@ -228,6 +240,63 @@ struct Tuned {
     int K, ww, grid;
 };
 std::map<std::string, Tuned> g_tuneCache;  // signature -> tuned options
+
+// Autotune results persist across processes in a text file, one line per batch signature:
+// "<64-bit FNV-1a of the signature, hex> <K> <wave width> <grid>".  The file is
+// $DVR_AUTOTUNE_CACHE, else $XDG_CACHE_HOME/diffdvr_amd/autotune.txt, else
+// $HOME/.cache/diffdvr_amd/autotune.txt; DVR_AUTOTUNE_CACHE set to an empty string turns it off.
+// Lines are appended (a later line for the same signature wins); read and write failures only
+// disable the persistence.
+std::string tune_file()
+{
+    if (const char* e = std::getenv("DVR_AUTOTUNE_CACHE")) return e;
+    std::string dir;
+    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) dir = x;
+    else if (const char* h = std::getenv("HOME"); h && *h) dir = std::string(h) + "/.cache";
+    else return "";
+    return dir + "/diffdvr_amd/autotune.txt";
+}
+
+uint64_t fnv1a(const std::string& s)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+bool load_tuned(const std::string& key, Tuned& out)
+{
+    const std::string f = tune_file();
+    if (f.empty()) return false;
+    FILE* fp = std::fopen(f.c_str(), "r");
+    if (!fp) return false;
+    const unsigned long long want = fnv1a(key);
+    unsigned long long h;
+    int K, ww, grid;
+    bool found = false;
+    while (std::fscanf(fp, "%llx %d %d %d", &h, &K, &ww, &grid) == 4)
+        if (h == want && (K == 8 || K == 16 || K == 32 || K == 64) && wave_shift_of(ww) >= 0 && grid > 0) {
+            out = Tuned{K, ww, grid};
+            found = true;
+        }
+    std::fclose(fp);
+    return found;
+}
+
+void store_tuned(const std::string& key, const Tuned& t)
+{
+    const std::string f = tune_file();
+    if (f.empty()) return;
+    for (size_t i = 1; i < f.size(); ++i)  // create the directory chain (failures show at fopen)
+        if (f[i] == '/') (void)mkdir(f.substr(0, i).c_str(), 0755);
+    FILE* fp = std::fopen(f.c_str(), "a");
+    if (!fp) return;
+    std::fprintf(fp, "%016llx %d %d %d\n", (unsigned long long)fnv1a(key), t.K, t.ww, t.grid);
+    std::fclose(fp);
+}
 
 void check_options(const dvr_batch_options* o, int& K, int& ww)
 {
@@ -386,14 +455,46 @@ dvr_batch_plan* create(const dvr_batch* b, const dvr_batch_options* o)
     std::vector<dvr::ZeroSeg> segs;
     p->dQueues = dev_alloc<unsigned>(dvr::kNumXcd);
     segs.push_back({reinterpret_cast<float*>(p->dQueues), dvr::kNumXcd});
+    // bricked volume-gradient scratch (2 x 2 x 4 voxels per 64-byte chunk), one range per gradient
+    // volume, each 256-byte aligned; cleared by every run, added into the gradient after the kernel
+    if (p->gm & 2) {
+        std::vector<size_t> off;
+        size_t total = 0;
+        for (auto& v : p->vols) {
+            off.push_back(total);
+            if (!v.adj) continue;
+            const size_t n = size_t((v.X + 1) / 2) * size_t((v.Y + 1) / 2) * size_t((v.Z + 3) / 4) * 16;
+            total += (n + 63) / 64 * 64;
+        }
+        if (total > 0) {
+            p->dBricks = dev_alloc<float>(total);
+            p->brickFloats = total;
+            std::vector<dvr::DebrickSeg> db;
+            for (size_t k = 0; k < p->vols.size(); ++k) {
+                dvr::BVol& v = p->vols[k];
+                if (!v.adj) continue;
+                v.brick = p->dBricks + off[k];
+                const long long nb = (long long)((v.X + 1) / 2) * ((v.Y + 1) / 2) * ((v.Z + 3) / 4);
+                segs.push_back({v.brick, nb * 16});
+                p->maxSegAlways = std::max(p->maxSegAlways, nb * 16);
+                db.push_back({v.brick, v.adj, v.X, v.Y, v.Z, 0});
+                p->maxBricks = std::max(p->maxBricks, nb);
+            }
+            p->nDebrick = (int)db.size();
+            p->dDebrick = dev_alloc<dvr::DebrickSeg>(db.size());
+            check_hip(hipMemcpy(p->dDebrick, db.data(), db.size() * sizeof(dvr::DebrickSeg), hipMemcpyHostToDevice),
+                      "hipMemcpy");
+        }
+    }
+    p->nSegsAlways = (int)segs.size();
     std::vector<dvr::ZeroSeg> g;
     for (auto& v : p->vols) if (v.adj) g.push_back({v.adj, (long long)v.X * v.Y * v.Z});
     for (auto& t : p->tfs) if (t.adj) g.push_back({t.adj, (long long)t.R * t.C});
     for (auto& v : p->views) if (v.acam) g.push_back({v.acam, 9});
     if (p->lossOut) g.push_back({p->lossOut, 1});
     std::sort(g.begin(), g.end(), [](const dvr::ZeroSeg& a, const dvr::ZeroSeg& c) { return a.p < c.p; });
-    for (auto& s : g) {  // merge duplicates and adjacent ranges
-        if (!segs.empty() && segs.size() > 1) {
+    for (auto& s : g) {  // merge duplicates and adjacent ranges (not into the always-cleared ranges)
+        if ((int)segs.size() > p->nSegsAlways) {
             dvr::ZeroSeg& l = segs.back();
             if (s.p <= l.p + l.n) {
                 const long long end = std::max<long long>(l.n, (long long)(s.p - l.p) + s.n);
@@ -548,6 +649,7 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
             std::lock_guard<std::mutex> lk(g_tuneMu);
             auto it = g_tuneCache.find(tuneKey);
             if (it != g_tuneCache.end()) best = it->second;
+            else if (load_tuned(tuneKey, best)) g_tuneCache[tuneKey] = best;
         }
         if (best.K == 0) {
             const int its = std::max(1, (int)iterations);
@@ -606,6 +708,7 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
             (void)hipEventDestroy(e1);
             std::lock_guard<std::mutex> lk(g_tuneMu);
             g_tuneCache[tuneKey] = best;
+            store_tuned(tuneKey, best);
         }
         check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
         dvr_batch_options o{best.K, best.ww, best.grid, plan->reinterpolate ? 1 : 0};
@@ -623,7 +726,8 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
 int64_t dvr_batch_plan_workspace_bytes(const dvr_batch_plan* plan)
 {
     if (!plan) return -1;
-    return (int64_t)((plan->ckFloats + plan->dsFloats) * sizeof(float) + plan->views.size() * sizeof(dvr::BView) +
+    return (int64_t)((plan->ckFloats + plan->dsFloats + plan->brickFloats) * sizeof(float) +
+                     plan->nDebrick * sizeof(dvr::DebrickSeg) + plan->views.size() * sizeof(dvr::BView) +
                      plan->vols.size() * sizeof(dvr::BVol) + plan->tfs.size() * sizeof(dvr::BTf) +
                      plan->nSegsAll * sizeof(dvr::ZeroSeg) + dvr::kNumXcd * sizeof(unsigned));
 }

@@ -77,9 +77,10 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
     float* ds = ADJ && B.ds ? B.ds + size_t(blockIdx.x) * size_t(B.dsCap) * kBatchBlock + threadIdx.x : nullptr;
     const bool priv = B.usePriv != 0;
     // volume gradients: the wave's corner values are summed per 64-byte chunk in LDS and flushed
-    // at the end of every tile (each tile belongs to one view, hence one volume)
+    // into the volume's bricked scratch at the end of every tile (each tile belongs to one view,
+    // hence one volume); k_debrick adds the scratch into the caller's gradient after the launch
     constexpr bool kVolInst = (GM & 2) != 0;
-    using VS = std::conditional_t<kVolInst, VolAgg, VolDirect>;
+    using VS = std::conditional_t<kVolInst, VolAgg<BrickLayout>, VolDirect>;
     VS vs;
     const bool aggOn = kVolInst && B.wantVol && B.ldsAgg > 0;
     if constexpr (kVolInst) {
@@ -127,7 +128,10 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
         if constexpr (kVolInst) {
             // every lane binds (uniform): the flush hands table entries to all lanes, also to
             // lanes outside the image
-            if (aggOn && vol.adj != nullptr) vs.bind(vol.adj);
+            if (aggOn && vol.adj != nullptr) {
+                vs.bind(vol.brick);
+                vs.L.set(vol.X, vol.Y, vol.Z);
+            }
         }
         const int tl = tile - V.tileBegin;
         const int ty = tl / V.tilesX, tx = tl - ty * V.tilesX;
@@ -195,7 +199,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(GM 
                 march_reverse<FILTER, TFM, BLEND, K, GM, SinkTiled<TFM>, true, VS>(VV, T, r, n, adj_acc, ck,
                                                                                    kBatchBlock, B.ckCap, B.ckEvery,
                                                                                    sink, curWantTf,
-                                                         wantVol, wantCam, vs, vol.Y * vol.Z, vol.Z, 1,
+                                                         wantVol, wantCam, vs,
                                                          adj_start, adj_dir, adj_step, stored ? ds : nullptr,
                                                          kBatchBlock);
                 if constexpr ((GM & 4) != 0) {

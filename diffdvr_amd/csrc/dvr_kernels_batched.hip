@@ -35,4 +35,48 @@ hipError_t launch_batched(const BatchParams& P, int filter, int tfm, int blend, 
 #undef DVR_CALL
 }
 
+// Adds each volume's bricked gradient scratch (2 x 2 x 4 voxels per 64-byte
+// chunk, bricks z-fastest) into the caller's contiguous X x Y x Z gradient: a
+// thread per brick, consecutive threads on consecutive z bricks of one (x, y)
+// brick column, so reads and the four 16-byte row segments it updates are
+// contiguous across the wave.  blockIdx.y = volume.
+__global__ __launch_bounds__(256) void k_debrick(const DebrickSeg* segs)
+{
+    const DebrickSeg S = segs[blockIdx.y];
+    const int BY = (S.Y + 1) >> 1, BZ = (S.Z + 3) >> 2;
+    const long long nb = (long long)((S.X + 1) >> 1) * BY * BZ;
+    const bool vec = (S.Z & 3) == 0;
+    for (long long b = (long long)blockIdx.x * 256 + threadIdx.x; b < nb; b += (long long)gridDim.x * 256) {
+        const int zb = int(b % BZ);
+        const long long r = b / BZ;
+        const int yb = int(r % BY), xb = int(r / BY);
+        const float4* src = reinterpret_cast<const float4*>(S.brick) + b * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int x = 2 * xb + (q >> 1), y = 2 * yb + (q & 1);
+            if (x >= S.X || y >= S.Y) continue;
+            const float4 v = src[q];
+            float* g = S.adj + ((size_t)x * S.Y + y) * S.Z + 4 * zb;
+            if (vec) {
+                float4 o = *reinterpret_cast<float4*>(g);
+                o.x += v.x; o.y += v.y; o.z += v.z; o.w += v.w;
+                *reinterpret_cast<float4*>(g) = o;
+            } else {
+                const float vv[4] = {v.x, v.y, v.z, v.w};
+                for (int k = 0; k < 4 && 4 * zb + k < S.Z; ++k) g[k] += vv[k];
+            }
+        }
+    }
+}
+
+hipError_t launch_debrick(const DebrickSeg* dSegs, int nSegs, long long maxBricks, hipStream_t s)
+{
+    if (nSegs <= 0) return hipSuccess;
+    long long bx = (maxBricks + 255) / 256;
+    if (bx > 2048) bx = 2048;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(k_debrick, dim3(unsigned(bx), unsigned(nSegs)), dim3(256), 0, s, dSegs);
+    return hipGetLastError();
+}
+
 }  // namespace dvr

@@ -79,6 +79,8 @@ hipError_t launch_adam(const AdamSeg* dSegs, int nSegs, long long maxN, long lon
 // 1 TF, 3 volume (+TF), 7 volume + TF + camera.
 hipError_t launch_batched(const BatchParams& P, int filter, int tfm, int blend, int K, int gm, int grid, size_t lds,
                           hipStream_t s, int* occ);
+// adds every volume's bricked gradient scratch into its gradient (after launch_batched)
+hipError_t launch_debrick(const DebrickSeg* dSegs, int nSegs, long long maxBricks, hipStream_t s);
 
 // pre-shaded volumes (dvr_kernels_preshaded.hip): plain structure, inversion adjoint
 hipError_t launch_forward_preshaded(const RenderParams& P, int blend, hipStream_t s);

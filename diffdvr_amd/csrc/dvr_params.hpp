@@ -112,8 +112,16 @@ struct BView {
 struct BVol {
     const float* p;
     float* adj;                 // gradient (accumulated) or NULL
+    float* brick;               // with adj: bricked gradient scratch of the batched kernel (2x2x4-voxel chunks)
     int X, Y, Z, pad0;
     float bmin[3], bsize[3];
+};
+
+// Bricked scratch of one volume gradient, added into the caller's gradient after the batched kernel
+struct DebrickSeg {
+    const float* brick;
+    float* adj;
+    int X, Y, Z, pad0;
 };
 
 struct BTf {

@@ -233,9 +233,72 @@ constexpr int kAggProbe = 8;
 constexpr unsigned kAggFlushAt = kAggSlots / 2;
 constexpr unsigned kAggEmpty = 0xFFFFFFFFu;
 
+// Element offsets of a cell's 8 corners in a gradient layout: corner (a,b,c)
+// (bits x<<2 | y<<1 | z) lies at x[a] + y[b] + z[c].  Corners past the border
+// clamp to it, as the reference's trilinear fetch does (cell_offsets).
+struct CornerIdx {
+    unsigned x[2], y[2], z[2];
+};
+__device__ __forceinline__ unsigned corner_at(const CornerIdx& o, int k)
+{
+    return o.x[(k >> 2) & 1] + o.y[(k >> 1) & 1] + o.z[k & 1];
+}
+// the two corner coordinates of cell index i along an axis of n voxels
+__device__ __forceinline__ void cell_axis(int i, int n, int& c0, int& c1)
+{
+    c0 = iclamp(i, 0, n - 1);
+    c1 = unsigned(i) < unsigned(n - 1) ? i + 1 : c0;
+}
+
+// X x Y x Z gradient with element strides (the caller's tensor)
+struct LinearLayout {
+    int X, Y, Z, sx, sy, sz;
+    __device__ __forceinline__ CornerIdx cell(int ix, int iy, int iz) const
+    {
+        const CellOffsets o = cell_offsets(ix, iy, iz, X, Y, Z, sx, sy, sz);
+        const unsigned bx = unsigned(iclamp(ix, 0, X - 1)) * unsigned(sx);
+        const unsigned by = unsigned(iclamp(iy, 0, Y - 1)) * unsigned(sy);
+        const unsigned bz = unsigned(iclamp(iz, 0, Z - 1)) * unsigned(sz);
+        return CornerIdx{{bx, bx + o.dx}, {by, by + o.dy}, {bz, bz + o.dz}};
+    }
+    __device__ __forceinline__ unsigned point(int x, int y, int z) const
+    {
+        return unsigned(x) * unsigned(sx) + unsigned(y) * unsigned(sy) + unsigned(z) * unsigned(sz);
+    }
+};
+
+// Bricked gradient scratch: every 64-byte chunk holds a 2 x 2 x 4 brick of
+// voxels (x, y, z; lane (x&1)<<3 | (y&1)<<2 | z&3), bricks z-fastest.  A ray's
+// corners -- and those of its neighbours, in any direction -- then fall into
+// fewer chunks than along 16-voxel z rows, so the chunk table flushes fewer
+// device requests.  k_debrick adds the scratch into the caller's gradient.
+struct BrickLayout {
+    int X, Y, Z;
+    unsigned sxb, syb;  // floats per brick column step in x (BY * BZ * 16) and y (BZ * 16)
+    __device__ __forceinline__ void set(int X_, int Y_, int Z_)
+    {
+        X = X_; Y = Y_; Z = Z_;
+        syb = unsigned((Z + 3) >> 2) * 16u;
+        sxb = unsigned((Y + 1) >> 1) * syb;
+    }
+    __device__ __forceinline__ unsigned xp(int x) const { return unsigned(x >> 1) * sxb + (unsigned(x & 1) << 3); }
+    __device__ __forceinline__ unsigned yp(int y) const { return unsigned(y >> 1) * syb + (unsigned(y & 1) << 2); }
+    __device__ __forceinline__ unsigned zp(int z) const { return (unsigned(z >> 2) << 4) + unsigned(z & 3); }
+    __device__ __forceinline__ CornerIdx cell(int ix, int iy, int iz) const
+    {
+        int x0, x1, y0, y1, z0, z1;
+        cell_axis(ix, X, x0, x1);
+        cell_axis(iy, Y, y0, y1);
+        cell_axis(iz, Z, z0, z1);
+        return CornerIdx{{xp(x0), xp(x1)}, {yp(y0), yp(y1)}, {zp(z0), zp(z1)}};
+    }
+    __device__ __forceinline__ unsigned point(int x, int y, int z) const { return xp(x) + yp(y) + zp(z); }
+};
+
 struct VolDirect {
     static constexpr bool kAgg = false;
     float* p;
+    LinearLayout L;
     __device__ __forceinline__ void add(unsigned off, float v)
     {
         if (v != 0.f) gatomic_add(p, off, v);
@@ -248,8 +311,10 @@ struct VolDirect {
     __device__ __forceinline__ void maybe_flush() {}
 };
 
+template <class Layout>
 struct VolAgg {
     static constexpr bool kAgg = true;
+    Layout L;         // element offsets of the gradient the table adds into
     unsigned* tags;   // [kAggSlots] chunk index, kAggEmpty = free
     unsigned* list;   // [kAggSlots] occupied slots in claim order
     unsigned* ctr;    // number of occupied slots
@@ -345,7 +410,7 @@ struct CornerCache {
 // leaving corners' offsets and values go to lo/lv[0..3] (pairs along z unless
 // B is z), the shared corners' state moves over, the fresh corners start empty.
 template <int B, bool UP, bool VOL, bool POS>
-__device__ __forceinline__ void corner_shift(CornerCache& c, const Vol& V, const CellOffsets& oldG,
+__device__ __forceinline__ void corner_shift(CornerCache& c, const Vol& V, const CornerIdx& oldG,
                                              const CellOffsets& newV, unsigned (&lo)[8], float (&lv)[8])
 {
     int m = 0;
@@ -355,7 +420,7 @@ __device__ __forceinline__ void corner_shift(CornerCache& c, const Vol& V, const
         const int f = UP ? k : (k | B);   // leaves; takes the shared corner's state
         const int s = UP ? (k | B) : k;   // the new cell's fresh corner
         if constexpr (VOL) {
-            lo[m] = corner_off(oldG, f);
+            lo[m] = corner_at(oldG, f);
             lv[m] = c.w[f];
             c.w[f] = c.w[s];
             c.w[s] = 0.f;
@@ -369,13 +434,13 @@ __device__ __forceinline__ void corner_shift(CornerCache& c, const Vol& V, const
 }
 
 template <bool VOL, bool POS, class VS>
-__device__ __forceinline__ void corner_flush_all(CornerCache& c, const Vol& V, VS& vs, int avx, int avy, int avz)
+__device__ __forceinline__ void corner_flush_all(CornerCache& c, const Vol& V, VS& vs)
 {
     if constexpr (VOL) {
         if (c.cx != kInvalidCell) {
-            const CellOffsets o = cell_offsets(c.cx, c.cy, c.cz, V.X, V.Y, V.Z, avx, avy, avz);
+            const CornerIdx o = vs.L.cell(c.cx, c.cy, c.cz);
 #pragma unroll
-            for (int k = 0; k < 8; k += 2) vs.add2(corner_off(o, k), c.w[k], corner_off(o, k + 1), c.w[k + 1]);
+            for (int k = 0; k < 8; k += 2) vs.add2(corner_at(o, k), c.w[k], corner_at(o, k + 1), c.w[k + 1]);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) c.w[k] = 0.f;
@@ -386,8 +451,7 @@ __device__ __forceinline__ void corner_flush_all(CornerCache& c, const Vol& V, V
 // handed to the sink once, after them: the sink's insertion code then runs
 // once per cell change instead of once per direction case.
 template <bool VOL, bool POS, class VS>
-__device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs, int avx, int avy, int avz, int ix,
-                                            int iy, int iz)
+__device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs, int ix, int iy, int iz)
 {
     const int ddx = ix - c.cx, ddy = iy - c.cy, ddz = iz - c.cz;
     const bool valid = c.cx != kInvalidCell;
@@ -397,7 +461,8 @@ __device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs
 #pragma unroll
     for (int k = 0; k < 8; ++k) { lo[k] = 0u; lv[k] = 0.f; }
     if (unit) {
-        const CellOffsets oldG = cell_offsets(c.cx, c.cy, c.cz, V.X, V.Y, V.Z, avx, avy, avz);
+        CornerIdx oldG{};
+        if constexpr (VOL) oldG = vs.L.cell(c.cx, c.cy, c.cz);
         const CellOffsets newV = cell_offsets(ix, iy, iz, V.X, V.Y, V.Z, V.sx, V.sy, V.sz);
         if (ddx > 0) corner_shift<4, true, VOL, POS>(c, V, oldG, newV, lo, lv);
         else if (ddx < 0) corner_shift<4, false, VOL, POS>(c, V, oldG, newV, lo, lv);
@@ -408,9 +473,9 @@ __device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs
     } else {
         if constexpr (VOL) {
             if (valid) {
-                const CellOffsets o = cell_offsets(c.cx, c.cy, c.cz, V.X, V.Y, V.Z, avx, avy, avz);
+                const CornerIdx o = vs.L.cell(c.cx, c.cy, c.cz);
 #pragma unroll
-                for (int k = 0; k < 8; ++k) { lo[k] = corner_off(o, k); lv[k] = c.w[k]; }
+                for (int k = 0; k < 8; ++k) { lo[k] = corner_at(o, k); lv[k] = c.w[k]; }
             }
 #pragma unroll
             for (int k = 0; k < 8; ++k) c.w[k] = 0.f;
@@ -438,7 +503,7 @@ template <int FILTER, int TFM, int BLEND, int K, int GM, class Sink, bool SPLIT 
 __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const Ray& r, int n, f4 adj_acc,
                                               const float* ck, int ckS, int cap, int ckEvery, Sink& sink,
                                               bool wantTf,
-                                              bool wantVol, bool wantPos, VS& vs, int avx, int avy, int avz,
+                                              bool wantVol, bool wantPos, VS& vs,
                                               f3& adj_start, f3& adj_dir, float& adj_step,
                                               const float* dsr = nullptr, int dsS = 0)
 {
@@ -543,7 +608,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                         if (wantVol) {
                             int gx, gy, gz;
                             nearest_index(p, V.X, V.Y, V.Z, gx, gy, gz);
-                            const int goff = gx * avx + gy * avy + gz * avz;
+                            const int goff = int(vs.L.point(gx, gy, gz));
                             if (goff != nLast) {
                                 if (nLast >= 0) vs.add(unsigned(nLast), nAcc);
                                 nLast = goff;
@@ -554,9 +619,9 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                     } else {
                         const int ix = int(p.x), iy = int(p.y), iz = int(p.z);
                         if (ix != cc.cx || iy != cc.cy || iz != cc.cz) {
-                            if (wantVol && wantPos) corner_move<true, true>(cc, V, vs, avx, avy, avz, ix, iy, iz);
-                            else if (wantVol) corner_move<true, false>(cc, V, vs, avx, avy, avz, ix, iy, iz);
-                            else corner_move<false, true>(cc, V, vs, avx, avy, avz, ix, iy, iz);
+                            if (wantVol && wantPos) corner_move<true, true>(cc, V, vs, ix, iy, iz);
+                            else if (wantVol) corner_move<true, false>(cc, V, vs, ix, iy, iz);
+                            else corner_move<false, true>(cc, V, vs, ix, iy, iz);
                         }
                         const float fx = p.x - float(ix), fy = p.y - float(iy), fz = p.z - float(iz);
                         float w[8];
@@ -596,7 +661,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
             if constexpr (FILTER == 0) {
                 if (nLast >= 0) vs.add(unsigned(nLast), nAcc);
             } else {
-                corner_flush_all<true, false>(cc, V, vs, avx, avy, avz);
+                corner_flush_all<true, false>(cc, V, vs);
             }
         }
     }
@@ -645,8 +710,9 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
     for (int e = threadIdx.x; e < ng; e += kBlock) s_g[e] = 0.0f;
     // volume gradient: the wave's corner values are summed per 64-byte chunk in LDS
     constexpr bool kVolInst = (GM & 2) != 0;
-    using VS = std::conditional_t<kVolInst, VolAgg, VolDirect>;
+    using VS = std::conditional_t<kVolInst, VolAgg<LinearLayout>, VolDirect>;
     VS vs;
+    vs.L = LinearLayout{P.X, P.Y, P.Z, A.avSx, A.avSy, A.avSz};
     const bool aggOn = kVolInst && A.avol != nullptr;
     if constexpr (kVolInst) {
         if (aggOn) {
@@ -711,8 +777,8 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
         if constexpr (!kVolInst) vs.p = nullptr;
         march_reverse<FILTER, TFM, BLEND, K, GM, SinkTiled<TFM>, TFM != 3, VS>(V, T, r, n, adj_acc, ck, kBlock, CK.cap,
                                                                                CK.every, sink, wantTf, wantVol,
-                                                 wantPos, vs, A.avSx, A.avSy,
-                                                 A.avSz, adj_start, adj_dir, adj_step, stored ? ds : nullptr, kBlock);
+                                                 wantPos, vs, adj_start, adj_dir, adj_step, stored ? ds : nullptr,
+                                                 kBlock);
         if (A.ast) {
             float* o = A.ast + b * A.astSb + y * A.astSy + x * A.astSx;
             if (A.stepBroadcast) atomicAdd(o, adj_step);

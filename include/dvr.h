@@ -454,8 +454,13 @@ dvr_status dvr_batch_plan_set_options(dvr_batch_plan* plan, const dvr_batch_opti
  * runs each, HIP events) in three stages -- workgroups per CU (1..4, unless
  * options.grid was set), wave tile width {8,16,4,32}, checkpoint interval
  * {8,16,32,64} -- keeps the fastest and writes it to *chosen.  Results are
- * cached per device and batch signature for the process.  Leaves outputs
- * overwritten and gradients ZEROED. */
+ * cached per device and batch signature (GPU architecture and compute units,
+ * modes, gradients, every volume / TF / view shape and step) for the process
+ * and in a text file: $DVR_AUTOTUNE_CACHE, else
+ * $XDG_CACHE_HOME/diffdvr_amd/autotune.txt, else
+ * $HOME/.cache/diffdvr_amd/autotune.txt (DVR_AUTOTUNE_CACHE="" turns the file
+ * off).  A cached result is applied without timing runs.  Timing runs
+ * overwrite the outputs; the gradients are left ZEROED either way. */
 dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr_batch_options* chosen, void* stream);
 /* Bytes of device scratch the plan holds (checkpoints, tables). */
 int64_t dvr_batch_plan_workspace_bytes(const dvr_batch_plan* plan);
