@@ -341,9 +341,10 @@ def bench_c2(args):
     n_samples = float(term.sum().item())
     ms_batched = timed(batched_step, steps, stream)
     value = n_samples / (ms_batched * 1e-3)
-    # the batched launch alone (one call = clear kernel + persistent kernel), HIP events on its stream
-    k_ms = [timed(batched_step, 1, stream) for _ in range(max(3, steps // 2))]
-    kernel_ms = float(np.mean(k_ms))
+    if args.path == "persistent":   # the persistent kernel alone (HIP events around it on its stream)
+        kernel_ms = float(np.median([plan.kernel_ms() for _ in range(max(3, steps // 2))]))
+    else:   # one call = the gradient clear and the fused kernel
+        kernel_ms = float(np.mean([timed(batched_step, 1, stream) for _ in range(max(3, steps // 2))]))
 
     config = {"workload": "C2: 128^3 blob volume, 64x4 texture TF gradient, 8 views 256x256, step 0.25 voxel, "
                           "L1 loss, one persistent fwd+loss+checkpointed-adjoint launch"
@@ -427,6 +428,17 @@ def bench_c3(args):
     def step():
         plan.run(zero_gradients=True)
 
+    if args.pmc_probe:   # 3 batched runs and one 256 MiB device copy (counter calibration)
+        for _ in range(3):
+            step()
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": src.numel() * 4,
+                          "samples_per_step": float(sum(v.termination.sum().item() for v in views))}))
+        return None
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -436,7 +448,9 @@ def bench_c3(args):
     config = {"workload": "C3: 256^3 blob volume + noise 0.02, identity TF x10, 64 Fibonacci views 256x256, "
                           "step 0.5 voxel, L2 vs a second volume, gradient w.r.t. all voxels, K autotuned",
               "views": B, "image": [H, W], "samples_per_step": n_samples, **plan.options}
-    result = result_line(args, "c3", value, ms, n_samples, config, ms,
+    # the persistent kernel alone (HIP events around it on the launch stream)
+    k_ms = float(np.median([plan.kernel_ms() for _ in range(5)]))
+    result = result_line(args, "c3", value, ms, n_samples, config, k_ms,
                          {"autotuned": tuned is not None})
     if not args.profile:
         inp = pr.RendererInputs()
@@ -529,7 +543,7 @@ def bench_c4(args):
     torch.cuda.synchronize()
     n_samples = float(sum(v.termination.sum().item() for v in views))
     ms = timed(step, steps, stream)
-    k_ms = float(np.mean([timed(lambda: plan.run(True), 1, stream) for _ in range(3)]))
+    k_ms = float(np.median([plan.kernel_ms() for _ in range(5)]))
     value = n_samples / (ms * 1e-3)
     config = {"workload": "C4: 128^3 blob volume, 4-component Gaussian TF, 512 sphere cameras of sizes "
                           "{64,128,256,512}^2, step 0.5 voxel, L2 vs perturbed-camera images, gradient w.r.t. "
