@@ -13,7 +13,9 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <optional>
 #include <string>
 #include <vector>
 
@@ -393,6 +395,63 @@ void* workspace(int device, size_t bytes, hipStream_t stream, int slot = 0)
     return w.ptr;
 }
 
+// Orders the uses of a device's library scratch (the workspace slots and the static
+// array of the allocation-free entry point) across streams and host threads.  A lease
+// holds the device's lock from before the scratch is handed out until the work that
+// uses it is enqueued; when the previous use was enqueued on another stream, the
+// calling stream first waits for the event recorded after it, and the lease records
+// that event again when it ends.  Under HIP graph capture neither happens: a captured
+// graph orders its own nodes, and its replays use the scratch in the order of the
+// stream they are launched on.
+struct ScratchSync {
+    std::mutex mu;
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;
+    bool used = false;
+};
+std::mutex g_sync_mutex;
+std::vector<std::unique_ptr<ScratchSync>> g_sync;
+
+class ScratchLease {
+public:
+    // with `device` current
+    ScratchLease(int device, hipStream_t s) : s_(s)
+    {
+        {
+            std::lock_guard<std::mutex> lk(g_sync_mutex);
+            if ((int)g_sync.size() <= device) g_sync.resize(device + 1);
+            if (!g_sync[device]) g_sync[device] = std::make_unique<ScratchSync>();
+            st_ = g_sync[device].get();
+        }
+        lk_ = std::unique_lock<std::mutex>(st_->mu);
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        check_hip(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+        capturing_ = cs != hipStreamCaptureStatusNone;
+        if (capturing_) return;
+        if (!st_->ev) check_hip(hipEventCreateWithFlags(&st_->ev, hipEventDisableTiming), "hipEventCreate");
+        if (st_->used && st_->last != s) check_hip(hipStreamWaitEvent(s, st_->ev, 0), "hipStreamWaitEvent");
+    }
+    ~ScratchLease()
+    {
+        if (capturing_ || !st_->ev) return;
+        if (hipEventRecord(st_->ev, s_) == hipSuccess) {
+            st_->last = s_;
+            st_->used = true;
+        } else {
+            (void)hipGetLastError();
+            st_->used = false;
+        }
+    }
+    ScratchLease(const ScratchLease&) = delete;
+    ScratchLease& operator=(const ScratchLease&) = delete;
+
+private:
+    hipStream_t s_;
+    ScratchSync* st_ = nullptr;
+    std::unique_lock<std::mutex> lk_;
+    bool capturing_ = false;
+};
+
 
 static_assert(dvr::kMaxTfLds == DVR_MAX_TILED_TF_ENTRIES, "the LDS TF limit is part of the C-ABI");
 bool tiled_ok(const Checked& c) { return !c.preshaded && (int64_t)c.P.R * c.P.C <= dvr::kMaxTfLds; }
@@ -580,6 +639,7 @@ void do_adjoint(const dvr_inputs* inputs, const dvr_outputs* fwd, const dvr_tens
     dvr::CkptParams CK = make_ckpt(c, K);
     const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * CK.cap * dvr::kBlock;
     bool fromFwd = false;
+    std::optional<ScratchLease> lease;  // the library scratch, until the launch is enqueued
     if (opt && given(opt->checkpoints)) {
         const dvr_tensor& ck = opt->checkpoints;
         int64_t numel = 1;
@@ -598,6 +658,7 @@ void do_adjoint(const dvr_inputs* inputs, const dvr_outputs* fwd, const dvr_tens
     } else {
         if (opt && given(opt->densities))
             fail(DVR_ERR_INVALID_ARGUMENT, "densities are read together with the checkpoints of the same forward");
+        lease.emplace(c.device, s);
         CK.buf = (float*)workspace(c.device, size_t(need) * sizeof(float), s);
         if (double(density_elements(c)) * sizeof(float) <= kMaxDensityStore) {  // the replaying forward writes them
             CK.ds = (float*)workspace(c.device, size_t(density_elements(c)) * sizeof(float), s, 1);
@@ -647,6 +708,7 @@ void do_fused(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* lo
     DeviceGuard g(c.device);
     hipStream_t s = (hipStream_t)stream;
     const int64_t need = (int64_t)dvr::tiled_num_tiles(c.P) * CK.cap * dvr::kBlock;
+    ScratchLease lease(c.device, s);  // the scratch (either kind), until the launch is enqueued
     if (fixed) {
         // checkpoints in the device's static scratch; no density store (the reverse march
         // re-interpolates: the same densities)
@@ -771,8 +833,10 @@ void do_forward_gradients(const dvr_inputs* inputs, const dvr_forward_difference
     G.grad = (float*)g.data;
     G.gSb = (int)g.strides[0]; G.gSy = (int)g.strides[1]; G.gSx = (int)g.strides[2];
     G.gSd = (int)g.strides[3]; G.gSc = (int)g.strides[4];
+    std::optional<ScratchLease> lease;  // the TF entry table, until the launch is enqueued
     if (fd->has_tf_derivatives) {
         const std::vector<int> entry = tf_entry_table(fd->d_tf, D, s);
+        lease.emplace(c.device, s);
         int* dEntry = (int*)workspace(c.device, entry.size() * sizeof(int), s);
         check_hip(hipMemcpyAsync(dEntry, entry.data(), entry.size() * sizeof(int), hipMemcpyHostToDevice, s),
                   "hipMemcpyAsync(tf entry table)");
@@ -875,7 +939,11 @@ void do_fwd_to_adjoint(const dvr_tensor* fvp, const dvr_tensor* acp, const dvr_f
     if (!F.ast && !F.ars && !atf) fail(DVR_ERR_NO_GRADIENT, "At least one adjoint variable must be activated!");
     DeviceGuard guard(device);
     hipStream_t s = (hipStream_t)stream;
-    if (atf) F.sums = (float*)workspace(device, sizeof(float) * size_t(B) * D, s);
+    std::optional<ScratchLease> lease;  // the per-derivative sums, until the launch is enqueued
+    if (atf) {
+        lease.emplace(device, s);
+        F.sums = (float*)workspace(device, sizeof(float) * size_t(B) * D, s);
+    }
     check_hip(dvr::launch_forward_to_adjoint(F, dtf, dSb, dSr, dSc, atf, tSb, tSr, tSc, R, C, s),
               "forward variables to gradients launch");
 }

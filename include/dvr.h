@@ -149,10 +149,15 @@ typedef struct dvr_adjoint_options {
  * rendering, and the TF tables of the forward-mode entry points use device
  * buffers the library keeps per device (grown on demand: a call that needs a
  * larger one synchronises the device, frees the old buffer and allocates).
- * These calls on one device are ordered by their stream; overlapping calls
- * on different streams of one device race on the scratch.  Entry points given
- * every buffer by the caller (checkpoints, densities) and batch plans, which
- * own their scratch, have no such restriction.
+ * Calls on different streams of one device are ordered: a call on another
+ * stream than the previous user of the scratch makes its stream wait for an
+ * event recorded after that use, and calls from several host threads take
+ * turns until their work is enqueued.  Under HIP graph capture this is not
+ * tracked: a captured graph's uses of the scratch are ordered by the graph and
+ * by the stream it is replayed on only, so its replays must not overlap other
+ * calls that use the scratch.  Entry points given every buffer by the caller
+ * (checkpoints, densities) and batch plans, which own their scratch, have no
+ * such restriction.
  */
 
 /* Thread-local message of the last failed call ("" if none). */
@@ -212,7 +217,8 @@ dvr_status dvr_render_fused(const dvr_inputs* inputs, dvr_outputs* outputs, cons
  * (dvr_checkpoint_buffer_elements) return DVR_ERR_INVALID_ARGUMENT: use a larger
  * checkpoint_interval or fewer pixels per call.  The first call on a device sets the array up
  * (hipGetSymbolAddress, which makes the runtime load it): make that call before capturing a graph.
- * Calls that overlap on one GPU (other streams) share the array and race: serialise them. */
+ * Calls on other streams of the GPU are ordered as for the library scratch above; replays of a
+ * captured graph are not, and race with calls that overlap them on that GPU. */
 #define DVR_NOALLOC_SCRATCH_FLOATS (8 * 1024 * 1024)
 dvr_status dvr_render_fused_noalloc(const dvr_inputs* inputs, dvr_outputs* outputs, const dvr_loss* loss,
                                     dvr_adjoint_outputs* adj_outputs, int32_t checkpoint_interval, void* stream);

@@ -312,3 +312,52 @@ def test_checkpoint_handover_is_checked():
     pr.Renderer.render_adjoint(inp, outs, g, adj, method="checkpoint", checkpoint_interval=16)
     torch.cuda.synchronize()
     assert float((a1 - adj.adj_tf).abs().max()) <= 1e-5 * float(a1.abs().max())
+
+
+@pytest.mark.gpu
+def test_scratch_ordered_across_streams():
+    """render_fused keeps its checkpoints and density store in scratch the library holds per
+    device; calls enqueued back to back on two streams must not overwrite each other's scratch
+    (the second stream waits for the first use), so both equal their sequential results."""
+    rng = np.random.default_rng(57)
+    scenes_ = [camera_scene(rng, 192, 160, tf_mode=1, camera=2, X=48, Y=40, Z=56, step=0.004, smooth=True),
+               camera_scene(rng, 40, 32, tf_mode=1, camera=2, X=20, Y=24, Z=16, step=0.01, smooth=True)]
+    inps = [gpu_inputs(s) for s in scenes_]
+    outs, adjs, targets = [], [], []
+    for s, inp in zip(scenes_, inps):
+        H, W = int(s["H"]), int(s["W"])
+        outs.append(pr.RendererOutputs(torch.empty(1, H, W, 4, device=dev()),
+                                       torch.empty(1, H, W, dtype=torch.int32, device=dev())))
+        a = pr.AdjointOutputs()
+        a.has_tf_derivatives = True
+        a.adj_tf = torch.zeros_like(inp.tf)
+        a.has_volume_derivatives = True
+        a.adj_volume = torch.zeros_like(inp.volume)
+        adjs.append(a)
+        targets.append(torch.rand(1, H, W, 4, device=dev(), generator=torch.Generator(device=dev()).manual_seed(5)))
+
+    def fused(k):
+        pr.Renderer.render_fused(inps[k], outs[k], "l2", targets[k], adjs[k], checkpoint_interval=16)
+
+    ref = []
+    for k in range(2):
+        fused(k)
+        torch.cuda.synchronize()
+        ref.append((outs[k].color.clone(), adjs[k].adj_tf.clone(), adjs[k].adj_volume.clone()))
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for rep in range(3):
+        for a in adjs:
+            a.adj_tf.zero_()
+            a.adj_volume.zero_()
+        torch.cuda.synchronize()
+        for k in (0, 1):
+            with torch.cuda.stream(streams[k]):
+                fused(k)
+        torch.cuda.synchronize()
+        for k in range(2):
+            c, gt, gv = ref[k]
+            assert torch.equal(outs[k].color, c), (rep, k)
+            # the gradients are sums of float atomics, equal up to their order
+            etf = rel_err(adjs[k].adj_tf.cpu().numpy(), gt.cpu().numpy())
+            evol = rel_err(adjs[k].adj_volume.cpu().numpy(), gv.cpu().numpy())
+            assert etf <= 1e-5 and evol <= 1e-5, (rep, k, etf, evol)
