@@ -417,6 +417,21 @@ class CameraOnASphere(nn.Module):
     def forward(self, center, yaw, pitch, distance):
         return pyrenderer.Camera.viewport_from_sphere(center, yaw, pitch, distance, self._orientation)
 
+    @staticmethod
+    def random_points(N: int, min_zoom=6, max_zoom=7, center=None):
+        """N cameras on the sphere (pytests/diffdvr/parametrizations.py:65-97): directions uniform on
+        the unit sphere (normalised Gaussian vectors, drawn from numpy's global generator), as
+        pitch = arccos(z) and yaw = atan2(y, x); distance = ZoomBase ** U(min_zoom, max_zoom).
+        Returns float32 tensors (center N*3, yaw N*1, pitch N*1, distance N*1)."""
+        v = np.random.randn(N, 3)
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        zoom = np.random.uniform(min_zoom, max_zoom, (N,))
+        dtype = renderer_dtype_torch
+        col = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dtype=dtype).unsqueeze(1)
+        c = torch.tensor([list(center) if center is not None else [0.0, 0.0, 0.0]] * N, dtype=dtype)
+        return c, col(np.arctan2(v[:, 1], v[:, 0])), col(np.arccos(v[:, 2])), \
+            col(np.power(CameraOnASphere.ZoomBase, zoom))
+
 
 class TfTexture(nn.Module):
     """RGB through a sigmoid, opacity through a softplus (beta 1, threshold 20)."""
@@ -427,6 +442,16 @@ class TfTexture(nn.Module):
         rgb = torch.sigmoid(tf_parameterized[:, :, :3])
         alpha = F.softplus(tf_parameterized[:, :, 3:], TfTexture._softplus_beta, TfTexture._softplus_threshold)
         return torch.cat([rgb, alpha], dim=2)
+
+    @staticmethod
+    def init_from_points(tf_points, R: int) -> torch.Tensor:
+        """The unconstrained 1*R*4 tensor whose forward() is the R-entry texture of `tf_points`
+        (TFUtils.get_texture_tensor): inverse sigmoid of r, g, b, inverse softplus of the opacity
+        (pytests/diffdvr/parametrizations.py:177-192)."""
+        tf = pyrenderer.TFUtils.get_texture_tensor(tf_points, R)
+        return torch.cat([inverseSigmoid(tf[:, :, :3]),
+                          inverseSoftplus(tf[:, :, 3:], TfTexture._softplus_beta, TfTexture._softplus_threshold)],
+                         dim=2)
 
 
 class TfPiecewiseLinear(nn.Module):
