@@ -427,8 +427,31 @@ constexpr int tf_cols()
     return TFM == 0 ? 2 : (TFM == 1 ? 4 : (TFM == 2 ? 5 : 6));
 }
 
-template <int TFM>
-__device__ __forceinline__ f4 tf_eval(const Tf& T, float density)
+// Interval of a piecewise-linear TF that holds a density: rows i and i+1, their
+// control-point positions, and (FAST) the reciprocal of the interval width.
+// FAST instances (the tiled, batched and forward-mode kernels) multiply by
+// v_rcp_f32's reciprocal (1 ulp) instead of dividing; the plain kernels keep
+// the reference's divisions (renderer_tf.cuh:309,337-357).
+struct TfLoc {
+    int i;
+    float p0, p1, inv;
+};
+template <bool FAST>
+__device__ __forceinline__ TfLoc tf_linear_locate(const Tf& T, float density)
+{
+    int i;
+    for (i = 0; i < T.R - 2; ++i)
+        if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
+    TfLoc L;
+    L.i = i;
+    L.p0 = T.p[i * T.sr + 4 * T.sc];
+    L.p1 = T.p[(i + 1) * T.sr + 4 * T.sc];
+    L.inv = FAST ? __builtin_amdgcn_rcpf(L.p1 - L.p0) : 0.0f;
+    return L;
+}
+
+template <int TFM, bool FAST = false>
+__device__ __forceinline__ f4 tf_eval(const Tf& T, float density, const TfLoc* loc = nullptr)
 {
     if constexpr (TFM == 0) {  // identity (renderer_tf.cuh:62-73)
         const float d = fclamp(density, 0.0f, 1.0f);
@@ -443,14 +466,12 @@ __device__ __forceinline__ f4 tf_eval(const Tf& T, float density)
         return mk4(lerpf_(a[0], c[0], df), lerpf_(a[T.sc], c[T.sc], df), lerpf_(a[2 * T.sc], c[2 * T.sc], df),
                    lerpf_(a[3 * T.sc], c[3 * T.sc], df));
     } else if constexpr (TFM == 2) {  // piecewise linear (:294-313)
-        int i;
-        for (i = 0; i < T.R - 2; ++i)
-            if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
-        const float* a = T.p + i * T.sr;
+        const TfLoc L = loc ? *loc : tf_linear_locate<FAST>(T, density);
+        const float* a = T.p + L.i * T.sr;
         const float* c = a + T.sr;
-        const float p0 = a[4 * T.sc], p1 = c[4 * T.sc];
+        const float p0 = L.p0, p1 = L.p1;
         const float d2 = fclamp(density, p0, p1);
-        const float f = (d2 - p0) / (p1 - p0);
+        const float f = FAST ? (d2 - p0) * L.inv : (d2 - p0) / (p1 - p0);
         return mk4(lerpf_(a[0], c[0], f), lerpf_(a[T.sc], c[T.sc], f), lerpf_(a[2 * T.sc], c[2 * T.sc], f),
                    lerpf_(a[3 * T.sc], c[3 * T.sc], f));
     } else {  // sum of Gaussians, column 5 = sigma (:491-519)
@@ -466,7 +487,7 @@ __device__ __forceinline__ f4 tf_eval(const Tf& T, float density)
 }
 
 // Absorption channel of the TF only (the transmittance recompute needs no colour).
-template <int TFM>
+template <int TFM, bool FAST = false>
 __device__ __forceinline__ float tf_absorption(const Tf& T, float density)
 {
     if constexpr (TFM == 0) {
@@ -479,14 +500,11 @@ __device__ __forceinline__ float tf_absorption(const Tf& T, float density)
         const float c = T.p[iclamp(di + 1, 0, T.R - 1) * T.sr + 3 * T.sc];
         return lerpf_(a, c, df);
     } else if constexpr (TFM == 2) {
-        int i;
-        for (i = 0; i < T.R - 2; ++i)
-            if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
-        const float* a = T.p + i * T.sr;
+        const TfLoc L = tf_linear_locate<FAST>(T, density);
+        const float* a = T.p + L.i * T.sr;
         const float* c = a + T.sr;
-        const float p0 = a[4 * T.sc], p1 = c[4 * T.sc];
-        const float d2 = fclamp(density, p0, p1);
-        const float f = (d2 - p0) / (p1 - p0);
+        const float d2 = fclamp(density, L.p0, L.p1);
+        const float f = FAST ? (d2 - L.p0) * L.inv : (d2 - L.p0) / (L.p1 - L.p0);
         return lerpf_(a[3 * T.sc], c[3 * T.sc], f);
     } else {
         float o = 0.0f;
@@ -670,8 +688,9 @@ __device__ __forceinline__ f4 blend_invert(const f4& acc_out, const f4& s, float
 // so the same code feeds global atomics, LDS atomics or private arrays.
 // Returns d(density).  (renderer_tf.cuh:75-104, 173-220, 315-400, 562-614)
 // ---------------------------------------------------------------------------
-template <int TFM, bool WANT_TF, class Sink>
-__device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4& adj, Sink& sink)
+template <int TFM, bool WANT_TF, class Sink, bool FAST = false>
+__device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4& adj, Sink& sink,
+                                            const TfLoc* loc = nullptr)
 {
     if constexpr (TFM == 0) {
         const float dc = fclamp(density, 0.0f, 1.0f);
@@ -694,23 +713,29 @@ __device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4
         if constexpr (WANT_TF) sink.template add_pair<false>(i0, 1.0f - df, i1, df, adj, 0.0f, 0.0f);
         return adj_df * float(T.R);
     } else if constexpr (TFM == 2) {
-        int i;
-        for (i = 0; i < T.R - 2; ++i)
-            if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
+        const TfLoc L = loc ? *loc : tf_linear_locate<FAST>(T, density);
+        const int i = L.i;
         const float* a = T.p + i * T.sr;
         const float* c = a + T.sr;
-        const float p0 = a[4 * T.sc], p1 = c[4 * T.sc];
+        const float p0 = L.p0, p1 = L.p1;
         const float d2 = fclamp(density, p0, p1);
-        const float f = (d2 - p0) / (p1 - p0);
+        const float f = FAST ? (d2 - p0) * L.inv : (d2 - p0) / (p1 - p0);
         const float adj_f = adj.x * (c[0] - a[0]) + adj.y * (c[T.sc] - a[T.sc]) + adj.z * (c[2 * T.sc] - a[2 * T.sc]) +
                             adj.w * (c[3 * T.sc] - a[3 * T.sc]);
-        const float den = (p1 - p0) * (p1 - p0);
         if constexpr (WANT_TF) {
-            const float adj_p0 = adj_f * (d2 - p1) / den;
-            const float adj_p1 = adj_f * (p0 - d2) / den;
+            float adj_p0, adj_p1;
+            if constexpr (FAST) {
+                const float q = adj_f * L.inv * L.inv;
+                adj_p0 = q * (d2 - p1);
+                adj_p1 = q * (p0 - d2);
+            } else {
+                const float den = (p1 - p0) * (p1 - p0);
+                adj_p0 = adj_f * (d2 - p1) / den;
+                adj_p1 = adj_f * (p0 - d2) / den;
+            }
             sink.template add_pair<true>(i, 1.0f - f, i + 1, f, adj, adj_p0, adj_p1);
         }
-        return adj_f / (p1 - p0);  // passes through the clamp (:350-351)
+        return FAST ? adj_f * L.inv : adj_f / (p1 - p0);  // passes through the clamp (:350-351)
     } else {
         float adj_d = 0.0f;
         for (int i = 0; i < T.R; ++i) {

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_forward_gradients(RenderParams P, Fw
         const f3 gv = mul3(gd, r.invVoxel);
         const float gStep = float(i) * dot3(gd, r.dp);
         f4 ds, de[kFwdChunk];
-        const f4 s = tf_eval<TFM>(T, dens);
+        const f4 s = tf_eval<TFM, true>(T, dens);
         tf_deriv<TFM, kFwdChunk>(T, dens, ds, ent, de);
         float alpha, keep;
         alpha_keep<BLEND>(s.w, r.step, alpha, keep);

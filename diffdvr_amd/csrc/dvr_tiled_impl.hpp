@@ -83,7 +83,7 @@ __device__ __forceinline__ void march_forward(const Vol& V, const Tf& T, const R
         const float tn = r.t0 + float(i + 1) * r.step;
         const f3 pn = sample_pos(r, tn);
         pipe.request(V, pn);
-        blend_t<BLEND>(acc, tf_eval<TFM>(T, d), r.step);
+        blend_t<BLEND>(acc, tf_eval<TFM, true>(T, d), r.step);
         if (ck) {
             if (--countdown == 0) {
                 countdown = K;
@@ -222,7 +222,10 @@ __device__ __forceinline__ unsigned corner_off(const CellOffsets& o, int k)
 // values per distinct 64-byte chunk, so the wave's corner gradients are first
 // summed in an LDS table of chunks (open addressing on the chunk index:
 // ds_cmpst claims a slot, ds_add_f32 adds the value) and the table is flushed
-// with one 16-lane atomic group per occupied chunk.  A chunk that finds no free
+// with one 16-lane atomic group per occupied chunk.  The claims of one cell
+// change are all in flight before the first is waited for; a claim only counts
+// the slots taken (one LDS add per call, no return), and the flush lists the
+// occupied slots by scanning the tags.  A chunk that finds no free
 // slot within kAggProbe probes goes straight to device memory.  The table
 // belongs to one wave (one-wave workgroups) and to one gradient tensor: it is
 // flushed when it is half full, at the end of every tile and before its
@@ -308,6 +311,12 @@ struct VolDirect {
         add(o0, v0);
         add(o1, v1);
     }
+    template <int N>
+    __device__ __forceinline__ void add_n(const unsigned (&o)[N], const float (&v)[N])
+    {
+#pragma unroll
+        for (int k = 0; k < N; ++k) add(o[k], v[k]);
+    }
     __device__ __forceinline__ void maybe_flush() {}
 };
 
@@ -316,7 +325,7 @@ struct VolAgg {
     static constexpr bool kAgg = true;
     Layout L;         // element offsets of the gradient the table adds into
     unsigned* tags;   // [kAggSlots] chunk index, kAggEmpty = free
-    unsigned* list;   // [kAggSlots] occupied slots in claim order
+    unsigned* list;   // [kAggSlots] occupied slots, listed by the flush
     unsigned* ctr;    // number of occupied slots
     float* vals;      // [kAggSlots][16]
     float* abase;     // gradient base rounded down to 64 bytes (never dereferenced below the base)
@@ -340,17 +349,32 @@ struct VolAgg {
         abase = (float*)(a & ~63ull);
         boff = unsigned(a & 63ull) >> 2;
     }
-    __device__ __forceinline__ unsigned claim(unsigned q)
+    static __device__ __forceinline__ unsigned home(unsigned q) { return (q * 0x9E3779B1u) >> (32 - kAggLog2); }
+    // Resolves a claim whose home slot s returned `old`: linear probing only for
+    // the lanes whose slot holds another chunk.  Returns the slot (kAggEmpty: none
+    // free within kAggProbe probes) and sets fresh when this call took a free slot.
+    __device__ __forceinline__ unsigned settle(unsigned q, unsigned s, unsigned old, bool& fresh)
     {
-        unsigned s = (q * 0x9E3779B1u) >> (32 - kAggLog2);
-        unsigned old = atomicCAS(&tags[s], kAggEmpty, q);
-        // linear probing only for the (rare) lanes whose slot holds another chunk
         for (int p = 1; p < kAggProbe && old != kAggEmpty && old != q; ++p) {
             s = (s + 1) & (kAggSlots - 1);
             old = atomicCAS(&tags[s], kAggEmpty, q);
         }
-        if (old == kAggEmpty) list[atomicAdd(ctr, 1u)] = s;
-        else if (old != q) s = kAggEmpty;
+        fresh = old == kAggEmpty;
+        return (old == kAggEmpty || old == q) ? s : kAggEmpty;
+    }
+    // occupied-slot count: c (uniform over the active lanes) added by the first
+    // active lane, without a return value
+    __device__ __forceinline__ void count(unsigned c)
+    {
+        const unsigned long long act = __ballot(1);
+        if (c != 0u && __lane_id() == unsigned(__ffsll((long long)act) - 1)) atomicAdd(ctr, c);
+    }
+    __device__ __forceinline__ unsigned claim(unsigned q)
+    {
+        const unsigned s0 = home(q);
+        bool fresh;
+        const unsigned s = settle(q, s0, atomicCAS(&tags[s0], kAggEmpty, q), fresh);
+        count(unsigned(__popcll(__ballot(fresh))));
         return s;
     }
     __device__ __forceinline__ void put(unsigned s, unsigned e, float v)
@@ -364,18 +388,55 @@ struct VolAgg {
         const unsigned e = boff + off;
         put(claim(e >> 4), e, v);
     }
-    // two values, usually z-neighbours in one chunk (then one claim)
+    // N values in pairs (2k, 2k+1), usually z-neighbours in one chunk (then one
+    // claim per pair).  The home-slot claims of all pairs are issued before any
+    // of them is waited for, so a cell change costs one LDS round trip, not one
+    // per claim.
+    template <int N>
+    __device__ __forceinline__ void add_n(const unsigned (&o)[N], const float (&v)[N])
+    {
+        static_assert(N % 2 == 0, "pairs");
+        unsigned e[N], s[N], old[N];
+        bool need[N], cl[N], same[N / 2];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            e[k] = boff + o[k];
+            need[k] = v[k] != 0.f;
+        }
+#pragma unroll
+        for (int p = 0; p < N / 2; ++p) {
+            same[p] = (e[2 * p] >> 4) == (e[2 * p + 1] >> 4);
+            cl[2 * p] = need[2 * p] || (need[2 * p + 1] && same[p]);
+            cl[2 * p + 1] = need[2 * p + 1] && !same[p];
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            s[k] = home(e[k] >> 4);
+            old[k] = kAggEmpty;
+            if (cl[k]) old[k] = atomicCAS(&tags[s[k]], kAggEmpty, e[k] >> 4);
+        }
+        bool fresh[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            fresh[k] = false;
+            if (cl[k]) s[k] = settle(e[k] >> 4, s[k], old[k], fresh[k]);
+        }
+        unsigned nfresh = 0u;
+#pragma unroll
+        for (int k = 0; k < N; ++k) nfresh += unsigned(__popcll(__ballot(fresh[k])));
+        count(nfresh);
+#pragma unroll
+        for (int p = 0; p < N / 2; ++p)
+            if (same[p]) s[2 * p + 1] = s[2 * p];
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (need[k]) put(s[k], e[k], v[k]);
+    }
     __device__ __forceinline__ void add2(unsigned o0, float v0, unsigned o1, float v1)
     {
-        const unsigned e0 = boff + o0, e1 = boff + o1;
-        const bool same = (e0 >> 4) == (e1 >> 4);
-        const bool n0 = v0 != 0.f, n1 = v1 != 0.f;
-        unsigned s0 = kAggEmpty, s1 = kAggEmpty;
-        if (n0 || (n1 && same)) s0 = claim(e0 >> 4);
-        if (n1 && !same) s1 = claim(e1 >> 4);
-        if (same) s1 = s0;
-        if (n0) put(s0, e0, v0);
-        if (n1) put(s1, e1, v1);
+        const unsigned o[2] = {o0, o1};
+        const float v[2] = {v0, v1};
+        add_n<2>(o, v);
     }
     __device__ __forceinline__ void flush()
     {
@@ -383,7 +444,19 @@ struct VolAgg {
         const unsigned cnt = unsigned(__popcll(act));
         const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(act >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(act), 0u));
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        const unsigned n = *ctr;
+        // list the occupied slots (the claims only count them)
+        unsigned n = 0u;
+        for (unsigned t0 = 0; t0 < unsigned(kAggSlots); t0 += cnt) {
+            const unsigned t = t0 + rank;
+            const bool occ = t < unsigned(kAggSlots) && tags[t] != kAggEmpty;
+            const unsigned long long m = __ballot(occ);
+            if (occ) {
+                const unsigned below = __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
+                list[n + below] = t;
+            }
+            n += unsigned(__popcll(m));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         for (unsigned e = rank; e < n * 16u; e += cnt) {
             const unsigned s = list[e >> 4], j = e & 15u;
             const float v = vals[s * 16 + j];
@@ -439,8 +512,10 @@ __device__ __forceinline__ void corner_flush_all(CornerCache& c, const Vol& V, V
     if constexpr (VOL) {
         if (c.cx != kInvalidCell) {
             const CornerIdx o = vs.L.cell(c.cx, c.cy, c.cz);
+            unsigned lo[8];
 #pragma unroll
-            for (int k = 0; k < 8; k += 2) vs.add2(corner_at(o, k), c.w[k], corner_at(o, k + 1), c.w[k + 1]);
+            for (int k = 0; k < 8; ++k) lo[k] = corner_at(o, k);
+            vs.add_n(lo, c.w);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) c.w[k] = 0.f;
@@ -483,11 +558,15 @@ __device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs
         if constexpr (POS) cell_load(V, ix, iy, iz, c.d);
     }
     if constexpr (VOL) {
-        vs.add2(lo[0], lv[0], lo[1], lv[1]);
-        vs.add2(lo[2], lv[2], lo[3], lv[3]);
-        if (!unit) {
-            vs.add2(lo[4], lv[4], lo[5], lv[5]);
-            vs.add2(lo[6], lv[6], lo[7], lv[7]);
+        // one batch of claims per cell change: the four leaving corners of a move across
+        // one face, or all eight when any lane of the wave jumps further (the unit
+        // movers' upper four values are zero and claim nothing)
+        if (__any(!unit)) {
+            vs.add_n(lo, lv);
+        } else {
+            const unsigned lo4[4] = {lo[0], lo[1], lo[2], lo[3]};
+            const float lv4[4] = {lv[0], lv[1], lv[2], lv[3]};
+            vs.add_n(lo4, lv4);
         }
     }
     c.cx = ix; c.cy = iy; c.cz = iz;
@@ -541,7 +620,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 const float t = r.t0 + float(i) * r.step;
                 const float d = dsr ? *(const gfloat*)(dsr + i * dsS) : fetch<FILTER>(V, sample_pos(r, t), cc);
                 float alpha, keep;
-                alpha_keep<BLEND>(tf_absorption<TFM>(T, d), r.step, alpha, keep);
+                alpha_keep<BLEND>(tf_absorption<TFM, true>(T, d), r.step, alpha, keep);
                 Tr *= keep;
             }
         }
@@ -558,7 +637,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
             for (int j = 0; j < K; ++j) {
                 tin[j] = Tr;
                 float alpha, keep;
-                alpha_keep<BLEND>(tf_absorption<TFM>(T, dv[j]), r.step, alpha, keep);
+                alpha_keep<BLEND>(tf_absorption<TFM, true>(T, dv[j]), r.step, alpha, keep);
                 if constexpr (kKeepAK) { al[j] = alpha; kp[j] = keep; }
                 Tr *= keep;
             }
@@ -575,7 +654,7 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                     if (more) pipe.request(V, pn);
                     tin[j] = Tr;
                     float alpha, keep;
-                    alpha_keep<BLEND>(tf_absorption<TFM>(T, dv[j]), r.step, alpha, keep);
+                    alpha_keep<BLEND>(tf_absorption<TFM, true>(T, dv[j]), r.step, alpha, keep);
                     if constexpr (kKeepAK) { al[j] = alpha; kp[j] = keep; }
                     Tr *= keep;
                     if (more) pipe.advance();
@@ -589,16 +668,19 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
         auto walk = [&](int j) __attribute__((always_inline)) {
             const int i = i0 + j;
             const float t = r.t0 + float(i) * r.step;
-            const f4 s = tf_eval<TFM>(T, dv[j]);
+            // a linear TF's interval is located once for the value and the adjoint
+            TfLoc L{};
+            if constexpr (TFM == 2) L = tf_linear_locate<true>(T, dv[j]);
+            const f4 s = tf_eval<TFM, true>(T, dv[j], &L);
             f4 adj_s;
             float adj_st;
             if constexpr (kKeepAK) blend_adjoint_ak<BLEND>(tin[j], s, r.step, al[j], kp[j], adj_acc, adj_s, adj_st);
             else blend_adjoint_t<BLEND>(tin[j], s, r.step, adj_acc, adj_s, adj_st);
             float adj_d;
-            if (wantTf) adj_d = tf_adjoint<TFM, CT_TF>(T, dv[j], adj_s, sink);
+            if (wantTf) adj_d = tf_adjoint<TFM, CT_TF, Sink, true>(T, dv[j], adj_s, sink, &L);
             else {
                 SinkNone none;
-                adj_d = tf_adjoint<TFM, false>(T, dv[j], adj_s, none);
+                adj_d = tf_adjoint<TFM, false, SinkNone, true>(T, dv[j], adj_s, none, &L);
             }
             if constexpr (CT_VOL || CT_POS) {
                 if (wantVol || wantPos) {
@@ -645,15 +727,25 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 if ((j & 3) == 0 && wantVol) vs.maybe_flush();
             }
         };
-        bool full = false;
-        if constexpr (SPLIT) full = __all(cnt == K);
-        if (full) {
-#pragma unroll
-            for (int j = K - 1; j >= 0; --j) walk(j);
-        } else {
-#pragma unroll
+        // Instances with volume gradients walk the segment as a loop: their per-sample body (cell
+        // changes, chunk-table claims) is long, and unrolled K times it made the C5 instance
+        // 143 KB of code; rolled it is 35 KB at the same speed (register arrays indexed by j
+        // become selects).
+        if constexpr (CT_VOL) {
+#pragma unroll 1
             for (int j = K - 1; j >= 0; --j)
                 if (j < cnt) walk(j);
+        } else {
+            bool full = false;
+            if constexpr (SPLIT) full = __all(cnt == K);
+            if (full) {
+#pragma unroll
+                for (int j = K - 1; j >= 0; --j) walk(j);
+            } else {
+#pragma unroll
+                for (int j = K - 1; j >= 0; --j)
+                    if (j < cnt) walk(j);
+            }
         }
     }
     if constexpr (CT_VOL) {
