@@ -420,9 +420,10 @@ def bench_c3(args):
     views = reference_frame_views(bt, vps, [(W, H)] * B, 0, 0, wl.fov_y, wl.step, targets=targets)
     plan = bt.BatchPlan([bt.BatchVolume(vol, grad=adj_vol)], [bt.BatchTf(tf)], views, volume_filter_mode=1,
                         tf_mode=0, blend_mode=0, loss="l2",
-                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8)
+                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8,
+                        grid=args.grid)
     tuned = None
-    if not args.checkpoint_interval and not args.wave_width:
+    if not (args.checkpoint_interval or args.wave_width or args.grid):
         tuned = plan.autotune(iterations=1)
 
     def step():

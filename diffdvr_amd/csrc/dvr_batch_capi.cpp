@@ -73,6 +73,7 @@ int tf_cols(int tfm)
 struct dvr_batch_plan {
     int device = 0;
     int filter = 1, tfm = 0, blend = 0, gm = 0, lossKind = 0;
+    bool volGrad = false;  // some volume gradient is requested (the chunk table is needed)
     float earlyOut = 1.0f - 1e-5f;
     float* lossOut = nullptr;
     std::vector<dvr::BView> views;  // sorted by (tf, volume); tileBegin/tilesX set by layout()
@@ -166,11 +167,12 @@ struct dvr_batch_plan {
                           : (copy_stride() > 0 ? size_t(dvr::kTfCopies) * copy_stride() : size_t(maxRC)))
                : 0;
         lds = sizeof(float) * (size_t(maxRC) + ldsG);
-        // volume gradients: the per-wave table of 64-byte chunks follows the TF tables
+        // volume gradients: the per-wave table of 64-byte chunks follows the TF tables (not
+        // reserved for camera-only plans, which run the same instance without it)
         aggOffset = 0;
-        if (gm & 2) {
+        if ((gm & 2) && volGrad) {
             aggOffset = dvr::agg_offset(maxRC, (int)ldsG);
-            lds = sizeof(float) * size_t(aggOffset) + dvr::kAggBytes;
+            lds = sizeof(float) * size_t(aggOffset) + dvr::agg_bytes(dvr::batched_agg_log2(tfm, gm));
         }
         int occ = 0;
         check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, 0, lds, nullptr, &occ),
@@ -181,8 +183,9 @@ struct dvr_batch_plan {
         resident = std::max(1, occ) * cus;
         // default: two waves per SIMD (eight one-wave workgroups per CU; measured
         // best for gather-bound tiles on MI355X: fewer concurrent rays per CU keep
-        // their cells in L1), within residency
-        int g = gridReq > 0 ? gridReq : std::min(resident, 8 * cus);
+        // their cells in L1), within residency; volume-gradient instances are bound
+        // by the latency of their walk and fill the CU (three or four waves per SIMD)
+        int g = gridReq > 0 ? gridReq : (((gm & 2) && volGrad) ? resident : std::min(resident, 8 * cus));
         g = std::max(1, std::min(g, std::max(nTiles, 1)));
         grid = g;
         cap = std::max(1, std::min((int)std::ceil(maxSamples / K), 1 << 14));
@@ -446,6 +449,7 @@ dvr_batch_plan* create(const dvr_batch* b, const dvr_batch_options* o)
         if (!anyTfGrad && !anyVolGrad && !anyCamGrad && !p->lossOut)
             fail(DVR_ERR_NO_GRADIENT, "At least one adjoint variable must be activated!");
         p->gm = anyCamGrad ? 7 : (anyVolGrad ? 3 : 1);
+        p->volGrad = anyVolGrad;
     }
     p->usePriv = p->gm && b->tf_mode == DVR_TF_GAUSSIAN && p->maxRC <= 48;
 

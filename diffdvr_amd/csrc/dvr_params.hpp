@@ -14,11 +14,23 @@ constexpr int kBatchBlock = 64;   // batched persistent kernel: one wave per wor
 constexpr int kTfCopies = 8;      // batched kernel: copies of the LDS TF-gradient table, lane & 7 selects one
 constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kMaxTfLds = 4096;   // max R*C floats of a TF staged in LDS by the tiled kernels
-// LDS table of the volume-gradient aggregation (one per wave): 2^kAggLog2 slots
-// of one 64-byte chunk each, plus tags, slot list and counter
+// LDS table of the volume-gradient aggregation (one per wave): 2^log2 slots of
+// one 64-byte chunk each, plus tags, slot list and counter.  The per-call tiled
+// adjoint uses kAggLog2; the batched instances size theirs by TF mode
+// (batched_agg_log2).
 constexpr int kAggLog2 = 8;
-constexpr int kAggSlots = 1 << kAggLog2;
-constexpr size_t kAggBytes = size_t(kAggSlots) * (4 + 4 + 16 * 4) + 16;
+constexpr size_t agg_bytes(int log2) { return (size_t(1) << log2) * (4 + 4 + 16 * 4) + 16; }
+constexpr size_t kAggBytes = agg_bytes(kAggLog2);
+// Batched volume-gradient instances: waves per SIMD the compiler must fit
+// (registers) and the chunk table that lets that many one-wave workgroups share
+// a CU's 160 KiB of LDS.  Measured on C5 / C3 (linear / identity TF): the walk is
+// bound by latency at two waves per SIMD; four waves with 64 slots ran C5 in
+// 105 ms against 110 ms at three (128 slots) and 136 ms at two (256 slots); the
+// identity TF (every sample scatters; wide bundles) ran C3 best at three.
+// Instances that also carry the position adjoint (camera gradients) need more
+// registers and keep two waves per SIMD with the 256-slot table.
+constexpr int batched_vol_waves(int tfm, int gm) { return (gm & 4) ? 2 : (tfm == 0 ? 3 : 4); }
+constexpr int batched_agg_log2(int tfm, int gm) { return (gm & 4) ? kAggLog2 : (tfm == 0 ? 7 : 6); }
 // float offset of the table in dynamic LDS after used0 + used1 floats (16-byte aligned)
 constexpr int agg_offset(int used0, int used1) { return (used0 + used1 + 3) & ~3; }
 constexpr int kMaxSamples = 1 << 24;  // hard cap on samples per ray (termination guarantee)

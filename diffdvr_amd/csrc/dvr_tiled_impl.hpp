@@ -222,9 +222,8 @@ __device__ __forceinline__ unsigned corner_off(const CellOffsets& o, int k)
 // values per distinct 64-byte chunk, so the wave's corner gradients are first
 // summed in an LDS table of chunks (open addressing on the chunk index:
 // ds_cmpst claims a slot, ds_add_f32 adds the value) and the table is flushed
-// with one 16-lane atomic group per occupied chunk.  The claims of one cell
-// change are all in flight before the first is waited for; a claim only counts
-// the slots taken (one LDS add per call, no return), and the flush lists the
+// with one 16-lane atomic group per occupied chunk.  A claim only counts the
+// slots taken (one LDS add per call, no return), and the flush lists the
 // occupied slots by scanning the tags.  A chunk that finds no free
 // slot within kAggProbe probes goes straight to device memory.  The table
 // belongs to one wave (one-wave workgroups) and to one gradient tensor: it is
@@ -233,7 +232,6 @@ __device__ __forceinline__ unsigned corner_off(const CellOffsets& o, int k)
 // work is spread by rank among them), so it may be called from divergent code.
 // ---------------------------------------------------------------------------
 constexpr int kAggProbe = 8;
-constexpr unsigned kAggFlushAt = kAggSlots / 2;
 constexpr unsigned kAggEmpty = 0xFFFFFFFFu;
 
 // Element offsets of a cell's 8 corners in a gradient layout: corner (a,b,c)
@@ -320,26 +318,28 @@ struct VolDirect {
     __device__ __forceinline__ void maybe_flush() {}
 };
 
-template <class Layout>
+template <class Layout, int LOG2 = kAggLog2>
 struct VolAgg {
     static constexpr bool kAgg = true;
+    static constexpr int kSlots = 1 << LOG2;
+    static constexpr unsigned kFlushAt = unsigned(kSlots) / 2;
     Layout L;         // element offsets of the gradient the table adds into
-    unsigned* tags;   // [kAggSlots] chunk index, kAggEmpty = free
-    unsigned* list;   // [kAggSlots] occupied slots, listed by the flush
+    unsigned* tags;   // [kSlots] chunk index, kAggEmpty = free
+    unsigned* list;   // [kSlots] occupied slots, listed by the flush
     unsigned* ctr;    // number of occupied slots
-    float* vals;      // [kAggSlots][16]
+    float* vals;      // [kSlots][16]
     float* abase;     // gradient base rounded down to 64 bytes (never dereferenced below the base)
     unsigned boff;    // floats from abase to the gradient base
 
-    // every thread of the (one-wave) workgroup; lds is 16-byte aligned, kAggBytes long
+    // every thread of the (one-wave) workgroup; lds is 16-byte aligned, agg_bytes(LOG2) long
     __device__ __forceinline__ void init(void* lds)
     {
         tags = (unsigned*)lds;
-        list = tags + kAggSlots;
-        ctr = list + kAggSlots;
+        list = tags + kSlots;
+        ctr = list + kSlots;
         vals = (float*)(ctr + 4);
-        for (int e = threadIdx.x; e < kAggSlots; e += blockDim.x) tags[e] = kAggEmpty;
-        for (int e = threadIdx.x; e < kAggSlots * 16; e += blockDim.x) vals[e] = 0.f;
+        for (int e = threadIdx.x; e < kSlots; e += blockDim.x) tags[e] = kAggEmpty;
+        for (int e = threadIdx.x; e < kSlots * 16; e += blockDim.x) vals[e] = 0.f;
         if (threadIdx.x == 0) *ctr = 0u;
     }
     // the table must be empty (flushed) when the base changes
@@ -349,14 +349,14 @@ struct VolAgg {
         abase = (float*)(a & ~63ull);
         boff = unsigned(a & 63ull) >> 2;
     }
-    static __device__ __forceinline__ unsigned home(unsigned q) { return (q * 0x9E3779B1u) >> (32 - kAggLog2); }
+    static __device__ __forceinline__ unsigned home(unsigned q) { return (q * 0x9E3779B1u) >> (32 - LOG2); }
     // Resolves a claim whose home slot s returned `old`: linear probing only for
     // the lanes whose slot holds another chunk.  Returns the slot (kAggEmpty: none
     // free within kAggProbe probes) and sets fresh when this call took a free slot.
     __device__ __forceinline__ unsigned settle(unsigned q, unsigned s, unsigned old, bool& fresh)
     {
         for (int p = 1; p < kAggProbe && old != kAggEmpty && old != q; ++p) {
-            s = (s + 1) & (kAggSlots - 1);
+            s = (s + 1) & (kSlots - 1);
             old = atomicCAS(&tags[s], kAggEmpty, q);
         }
         fresh = old == kAggEmpty;
@@ -389,48 +389,25 @@ struct VolAgg {
         put(claim(e >> 4), e, v);
     }
     // N values in pairs (2k, 2k+1), usually z-neighbours in one chunk (then one
-    // claim per pair).  The home-slot claims of all pairs are issued before any
-    // of them is waited for, so a cell change costs one LDS round trip, not one
-    // per claim.
+    // claim per pair).  The pairs are claimed one after the other: issuing all
+    // claims of a cell change before waiting for the first measured the same
+    // time and held more registers.
     template <int N>
     __device__ __forceinline__ void add_n(const unsigned (&o)[N], const float (&v)[N])
     {
         static_assert(N % 2 == 0, "pairs");
-        unsigned e[N], s[N], old[N];
-        bool need[N], cl[N], same[N / 2];
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            e[k] = boff + o[k];
-            need[k] = v[k] != 0.f;
-        }
 #pragma unroll
         for (int p = 0; p < N / 2; ++p) {
-            same[p] = (e[2 * p] >> 4) == (e[2 * p + 1] >> 4);
-            cl[2 * p] = need[2 * p] || (need[2 * p + 1] && same[p]);
-            cl[2 * p + 1] = need[2 * p + 1] && !same[p];
+            const unsigned e0 = boff + o[2 * p], e1 = boff + o[2 * p + 1];
+            const bool same = (e0 >> 4) == (e1 >> 4);
+            const bool n0 = v[2 * p] != 0.f, n1 = v[2 * p + 1] != 0.f;
+            unsigned s0 = kAggEmpty, s1 = kAggEmpty;
+            if (n0 || (n1 && same)) s0 = claim(e0 >> 4);
+            if (n1 && !same) s1 = claim(e1 >> 4);
+            if (same) s1 = s0;
+            if (n0) put(s0, e0, v[2 * p]);
+            if (n1) put(s1, e1, v[2 * p + 1]);
         }
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            s[k] = home(e[k] >> 4);
-            old[k] = kAggEmpty;
-            if (cl[k]) old[k] = atomicCAS(&tags[s[k]], kAggEmpty, e[k] >> 4);
-        }
-        bool fresh[N];
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            fresh[k] = false;
-            if (cl[k]) s[k] = settle(e[k] >> 4, s[k], old[k], fresh[k]);
-        }
-        unsigned nfresh = 0u;
-#pragma unroll
-        for (int k = 0; k < N; ++k) nfresh += unsigned(__popcll(__ballot(fresh[k])));
-        count(nfresh);
-#pragma unroll
-        for (int p = 0; p < N / 2; ++p)
-            if (same[p]) s[2 * p + 1] = s[2 * p];
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            if (need[k]) put(s[k], e[k], v[k]);
     }
     __device__ __forceinline__ void add2(unsigned o0, float v0, unsigned o1, float v1)
     {
@@ -446,9 +423,9 @@ struct VolAgg {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         // list the occupied slots (the claims only count them)
         unsigned n = 0u;
-        for (unsigned t0 = 0; t0 < unsigned(kAggSlots); t0 += cnt) {
+        for (unsigned t0 = 0; t0 < unsigned(kSlots); t0 += cnt) {
             const unsigned t = t0 + rank;
-            const bool occ = t < unsigned(kAggSlots) && tags[t] != kAggEmpty;
+            const bool occ = t < unsigned(kSlots) && tags[t] != kAggEmpty;
             const unsigned long long m = __ballot(occ);
             if (occ) {
                 const unsigned below = __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
@@ -469,7 +446,7 @@ struct VolAgg {
     }
     __device__ __forceinline__ void maybe_flush()
     {
-        if (*ctr > kAggFlushAt) flush();
+        if (*ctr > kFlushAt) flush();
     }
 };
 
@@ -592,7 +569,10 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
     wantPos = CT_POS && wantPos;
     // Beer-Lambert segments of up to 16 samples also keep each sample's (alpha,
     // keep) from the recompute, so the backward walk evaluates no exponential
-    constexpr bool kKeepAK = BLEND == 0 && K <= 16;
+    // (not in the volume-gradient instances without the position adjoint: they run more waves
+    // per SIMD and need the registers)
+    constexpr bool kVolOnly = CT_VOL && !CT_POS;
+    constexpr bool kKeepAK = BLEND == 0 && K <= 16 && !kVolOnly;
     float dv[K], tin[K];
     float al[kKeepAK ? K : 1], kp[kKeepAK ? K : 1];
     // reverse-walk cell state for the volume / position adjoint
@@ -727,11 +707,11 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                 if ((j & 3) == 0 && wantVol) vs.maybe_flush();
             }
         };
-        // Instances with volume gradients walk the segment as a loop: their per-sample body (cell
-        // changes, chunk-table claims) is long, and unrolled K times it made the C5 instance
-        // 143 KB of code; rolled it is 35 KB at the same speed (register arrays indexed by j
-        // become selects).
-        if constexpr (CT_VOL) {
+        // Volume-gradient instances (without the position adjoint) walk the segment as a loop:
+        // their per-sample body (cell changes, chunk-table claims) is long, and unrolled K times
+        // it made the C5 instance 143 KB of code; rolled it is 35 KB at the same speed (register
+        // arrays indexed by j become selects) and needs fewer registers.
+        if constexpr (kVolOnly) {
 #pragma unroll 1
             for (int j = K - 1; j >= 0; --j)
                 if (j < cnt) walk(j);
