@@ -647,7 +647,8 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
         DeviceGuard g(plan->device);
         hipStream_t s = (hipStream_t)stream;
         const int gridReq = plan->gridReq;
-        const std::string tuneKey = plan->signature + (plan->reinterpolate ? "|reinterpolate" : "");
+        // the leading tag names the kernels' and search's revision: results of older ones are not reused
+        const std::string tuneKey = "r2|" + plan->signature + (plan->reinterpolate ? "|reinterpolate" : "");
         Tuned best{0, 0, 0};
         {
             std::lock_guard<std::mutex> lk(g_tuneMu);
@@ -675,33 +676,35 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
                 return ms;
             };
             try {
-                // staged search: workgroups per CU, then wave tile shape, then checkpoint interval
+                // joint search over checkpoint interval and workgroups per CU (each interval is
+                // its own instance, with its own registers and residency), then the wave tile shape
                 Tuned cur{16, 8, 0};
                 float bestMs = 1e30f;
-                {
-                    dvr_batch_options o{cur.K, cur.ww, 0, reint};
-                    set_options(plan, &o);
+                std::vector<int> Ks;
+                if (plan->gm) Ks = {8, 16, 32, 64};
+                else Ks = {16};
+                for (int K : Ks) {
+                    {
+                        dvr_batch_options o{K, 8, 0, reint};
+                        set_options(plan, &o);
+                    }
+                    const int cus = plan->cus, resident = plan->resident;
+                    std::vector<int> grids;
+                    if (gridReq > 0) grids.push_back(gridReq);
+                    else {
+                        for (int k = 1; k <= 4; ++k)  // 1..4 waves per SIMD
+                            if (4 * k * cus <= resident) grids.push_back(4 * k * cus);
+                        if (grids.empty() || grids.back() < resident) grids.push_back(resident);
+                    }
+                    for (int g : grids) {
+                        const float ms = time_it(K, 8, g);
+                        if (ms < bestMs) { bestMs = ms; cur = Tuned{K, 8, g}; }
+                    }
                 }
-                const int cus = plan->cus, resident = plan->resident;
-                std::vector<int> grids;
-                if (gridReq > 0) grids.push_back(gridReq);
-                else
-                    for (int k = 1; k <= 4; ++k)  // 1..4 waves per SIMD
-                        if (4 * k * cus <= resident) grids.push_back(4 * k * cus);
-                if (grids.empty()) grids.push_back(resident);
-                for (int g : grids) {
-                    const float ms = time_it(cur.K, cur.ww, g);
-                    if (ms < bestMs) { bestMs = ms; cur.grid = g; }
-                }
-                for (int ww : {8, 16, 4, 32}) {
+                for (int ww : {16, 4, 32}) {
                     const float ms = time_it(cur.K, ww, cur.grid);
                     if (ms < bestMs) { bestMs = ms; cur.ww = ww; }
                 }
-                if (plan->gm)
-                    for (int K : {8, 16, 32, 64}) {
-                        const float ms = time_it(K, cur.ww, cur.grid);
-                        if (ms < bestMs) { bestMs = ms; cur.K = K; }
-                    }
                 best = cur;
             } catch (...) {
                 (void)hipEventDestroy(e0);

@@ -26,10 +26,12 @@ constexpr size_t kAggBytes = agg_bytes(kAggLog2);
 // a CU's 160 KiB of LDS.  Measured on C5 / C3 (linear / identity TF): the walk is
 // bound by latency at two waves per SIMD; four waves with 64 slots ran C5 in
 // 105 ms against 110 ms at three (128 slots) and 136 ms at two (256 slots); the
-// identity TF (every sample scatters; wide bundles) ran C3 best at three.
-// Instances that also carry the position adjoint (camera gradients) need more
-// registers and keep two waves per SIMD with the 256-slot table.
-constexpr int batched_vol_waves(int tfm, int gm) { return (gm & 4) ? 2 : (tfm == 0 ? 3 : 4); }
+// identity TF (every sample scatters; wide bundles) needs the 128-slot table,
+// which its small TF tables leave room for at four waves (C3 19.6 ms at four,
+// 21.7 ms at three, 24.1 ms at two).  Instances that also carry the position
+// adjoint (camera gradients) need more registers and keep two waves per SIMD
+// with the 256-slot table.
+constexpr int batched_vol_waves(int tfm, int gm) { return (gm & 4) ? 2 : 4; }
 constexpr int batched_agg_log2(int tfm, int gm) { return (gm & 4) ? kAggLog2 : (tfm == 0 ? 7 : 6); }
 // float offset of the table in dynamic LDS after used0 + used1 floats (16-byte aligned)
 constexpr int agg_offset(int used0, int used1) { return (used0 + used1 + 3) & ~3; }

@@ -535,14 +535,15 @@ __device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs
         if constexpr (POS) cell_load(V, ix, iy, iz, c.d);
     }
     if constexpr (VOL) {
-        // one batch of claims per cell change: the four leaving corners of a move across
-        // one face, or all eight when any lane of the wave jumps further (the unit
-        // movers' upper four values are zero and claim nothing)
-        if (__any(!unit)) {
-            vs.add_n(lo, lv);
-        } else {
+        // the four leaving corners of a move across one face; all eight for a jump
+        {
             const unsigned lo4[4] = {lo[0], lo[1], lo[2], lo[3]};
             const float lv4[4] = {lv[0], lv[1], lv[2], lv[3]};
+            vs.add_n(lo4, lv4);
+        }
+        if (!unit) {
+            const unsigned lo4[4] = {lo[4], lo[5], lo[6], lo[7]};
+            const float lv4[4] = {lv[4], lv[5], lv[6], lv[7]};
             vs.add_n(lo4, lv4);
         }
     }

@@ -457,9 +457,10 @@ dvr_status dvr_batch_plan_get_options(const dvr_batch_plan* plan, dvr_batch_opti
 /* Re-layout for new options; may allocate (set-up only, not under capture). */
 dvr_status dvr_batch_plan_set_options(dvr_batch_plan* plan, const dvr_batch_options* options);
 /* Autotuner: times candidates on this plan's own batch (`iterations` timed
- * runs each, HIP events) in three stages -- workgroups per CU (1..4, unless
- * options.grid was set), wave tile width {8,16,4,32}, checkpoint interval
- * {8,16,32,64} -- keeps the fastest and writes it to *chosen.  Results are
+ * runs each, HIP events) in two stages -- checkpoint interval {8,16,32,64}
+ * jointly with workgroups per CU (1..4 waves per SIMD and full residency, per
+ * interval, unless options.grid was set), then wave tile width {8,16,4,32} --
+ * keeps the fastest and writes it to *chosen.  Results are
  * cached per device and batch signature (GPU architecture and compute units,
  * modes, gradients, every volume / TF / view shape and step) for the process
  * and in a text file: $DVR_AUTOTUNE_CACHE, else
