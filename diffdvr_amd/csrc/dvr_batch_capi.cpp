@@ -89,6 +89,7 @@ struct dvr_batch_plan {
     int qBegin[dvr::kNumXcd + 1] = {};
     size_t lds = 0;
     int aggOffset = 0;  // float offset of the volume-gradient chunk table in LDS (0: none)
+    int aggLog2 = dvr::kAggLog2;  // log2 of the chunk table's slots
     // device
     dvr::BView* dViews = nullptr;
     dvr::BVol* dVols = nullptr;
@@ -145,6 +146,7 @@ struct dvr_batch_plan {
         P.ldsTf = maxRC; P.usePriv = usePriv ? 1 : 0;
         P.tfCopyStride = (gm && !usePriv) ? copy_stride() : 0;
         P.ldsAgg = aggOffset;
+        P.aggLog2 = aggLog2;
         return P;
     }
 
@@ -170,13 +172,23 @@ struct dvr_batch_plan {
         // volume gradients: the per-wave table of 64-byte chunks follows the TF tables (not
         // reserved for camera-only plans, which run the same instance without it)
         aggOffset = 0;
-        if ((gm & 2) && volGrad) {
-            aggOffset = dvr::agg_offset(maxRC, (int)ldsG);
-            lds = sizeof(float) * size_t(aggOffset) + dvr::agg_bytes(dvr::batched_agg_log2(tfm, gm));
-        }
         int occ = 0;
-        check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, 0, lds, nullptr, &occ),
-                  "occupancy query");
+        if ((gm & 2) && volGrad) {
+            // the largest table that keeps the instance at the waves per SIMD its registers
+            // were fitted for (workgroups per CU from the runtime's occupancy calculator)
+            aggOffset = dvr::agg_offset(maxRC, (int)ldsG);
+            const int target = 4 * dvr::batched_vol_waves(gm);
+            for (int lg = dvr::kAggLog2; lg >= dvr::kAggMinLog2; --lg) {
+                aggLog2 = lg;
+                lds = sizeof(float) * size_t(aggOffset) + dvr::agg_bytes(lg);
+                check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, 0, lds, nullptr, &occ),
+                          "occupancy query");
+                if (occ >= target) break;
+            }
+        } else {
+            check_hip(dvr::launch_batched(params(), filter, tfm, blend, segK(), gm, 0, lds, nullptr, &occ),
+                      "occupancy query");
+        }
         hipDeviceProp_t prop;
         check_hip(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         cus = prop.multiProcessorCount;
@@ -184,7 +196,7 @@ struct dvr_batch_plan {
         // default: two waves per SIMD (eight one-wave workgroups per CU; measured
         // best for gather-bound tiles on MI355X: fewer concurrent rays per CU keep
         // their cells in L1), within residency; volume-gradient instances are bound
-        // by the latency of their walk and fill the CU (three or four waves per SIMD)
+        // by the latency of their walk and fill the CU (four waves per SIMD)
         int g = gridReq > 0 ? gridReq : (((gm & 2) && volGrad) ? resident : std::min(resident, 8 * cus));
         g = std::max(1, std::min(g, std::max(nTiles, 1)));
         grid = g;

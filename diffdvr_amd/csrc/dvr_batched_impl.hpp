@@ -61,7 +61,7 @@ __device__ __forceinline__ int batched_pop(const BatchParams& B, int xcd, int& q
 
 template <int FILTER, int TFM, int BLEND, int K, int GM>
 __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(
-    GM == 0 ? 5 : ((GM & 2) ? batched_vol_waves(TFM, GM) : 2)))) void k_batched(BatchParams B)
+    GM == 0 ? 5 : ((GM & 2) ? batched_vol_waves(GM) : 2)))) void k_batched(BatchParams B)
 {
     constexpr bool ADJ = GM != 0;
     extern __shared__ float lds[];
@@ -81,11 +81,11 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(
     // into the volume's bricked scratch at the end of every tile (each tile belongs to one view,
     // hence one volume); k_debrick adds the scratch into the caller's gradient after the launch
     constexpr bool kVolInst = (GM & 2) != 0;
-    using VS = std::conditional_t<kVolInst, VolAgg<BrickLayout, batched_agg_log2(TFM, GM)>, VolDirect>;
+    using VS = std::conditional_t<kVolInst, VolAgg<BrickLayout>, VolDirect>;
     VS vs;
     const bool aggOn = kVolInst && B.wantVol && B.ldsAgg > 0;
     if constexpr (kVolInst) {
-        if (aggOn) vs.init(lds + B.ldsAgg);
+        if (aggOn) vs.init(lds + B.ldsAgg, B.aggLog2);
     } else {
         vs.p = nullptr;
     }

@@ -16,23 +16,21 @@ constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round
 constexpr int kMaxTfLds = 4096;   // max R*C floats of a TF staged in LDS by the tiled kernels
 // LDS table of the volume-gradient aggregation (one per wave): 2^log2 slots of
 // one 64-byte chunk each, plus tags, slot list and counter.  The per-call tiled
-// adjoint uses kAggLog2; the batched instances size theirs by TF mode
-// (batched_agg_log2).
+// adjoint uses kAggLog2; a batched plan takes the largest table (kAggMinLog2 ..
+// kAggLog2) that keeps its instance at batched_vol_waves waves per SIMD.
 constexpr int kAggLog2 = 8;
+constexpr int kAggMinLog2 = 5;
 constexpr size_t agg_bytes(int log2) { return (size_t(1) << log2) * (4 + 4 + 16 * 4) + 16; }
 constexpr size_t kAggBytes = agg_bytes(kAggLog2);
 // Batched volume-gradient instances: waves per SIMD the compiler must fit
-// (registers) and the chunk table that lets that many one-wave workgroups share
-// a CU's 160 KiB of LDS.  Measured on C5 / C3 (linear / identity TF): the walk is
-// bound by latency at two waves per SIMD; four waves with 64 slots ran C5 in
-// 105 ms against 110 ms at three (128 slots) and 136 ms at two (256 slots); the
-// identity TF (every sample scatters; wide bundles) needs the 128-slot table,
-// which its small TF tables leave room for at four waves (C3 19.6 ms at four,
-// 21.7 ms at three, 24.1 ms at two).  Instances that also carry the position
-// adjoint (camera gradients) need more registers and keep two waves per SIMD
-// with the 256-slot table.
-constexpr int batched_vol_waves(int tfm, int gm) { return (gm & 4) ? 2 : 4; }
-constexpr int batched_agg_log2(int tfm, int gm) { return (gm & 4) ? kAggLog2 : (tfm == 0 ? 7 : 6); }
+// (registers).  Measured on C5 / C3 (linear / identity TF): the walk is bound by
+// latency at two waves per SIMD; four waves ran C5 in 105 ms (64-slot table)
+// against 110 ms at three (128 slots) and 136 ms at two (256 slots), C3 in
+// 19.6 ms (128 slots) against 21.7 / 24.1 ms, and C3 with a 64-slot table in
+// 27 ms: the table shrinks only as far as the TF tables beside it require.
+// Instances that also carry the position adjoint (camera gradients) need more
+// registers and keep two waves per SIMD.
+constexpr int batched_vol_waves(int gm) { return (gm & 4) ? 2 : 4; }
 // float offset of the table in dynamic LDS after used0 + used1 floats (16-byte aligned)
 constexpr int agg_offset(int used0, int used1) { return (used0 + used1 + 3) & ~3; }
 constexpr int kMaxSamples = 1 << 24;  // hard cap on samples per ray (termination guarantee)
@@ -165,6 +163,7 @@ struct BatchParams {
     int usePriv;                // Gaussian TF gradient in per-thread LDS slices
     int tfCopyStride;           // floats between the kTfCopies TF-gradient tables (0: one table)
     int ldsAgg;                 // float offset of the volume-gradient chunk table in LDS (0: none)
+    int aggLog2;                // log2 of the chunk table's slots
 };
 
 // One segment of a zero-fill list (gradients, counters) cleared by a kernel

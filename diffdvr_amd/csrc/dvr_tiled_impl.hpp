@@ -318,22 +318,24 @@ struct VolDirect {
     __device__ __forceinline__ void maybe_flush() {}
 };
 
-template <class Layout, int LOG2 = kAggLog2>
+template <class Layout>
 struct VolAgg {
     static constexpr bool kAgg = true;
-    static constexpr int kSlots = 1 << LOG2;
-    static constexpr unsigned kFlushAt = unsigned(kSlots) / 2;
+    unsigned lg;      // log2 of the slot count (uniform)
     Layout L;         // element offsets of the gradient the table adds into
-    unsigned* tags;   // [kSlots] chunk index, kAggEmpty = free
-    unsigned* list;   // [kSlots] occupied slots, listed by the flush
+    unsigned* tags;   // [slots] chunk index, kAggEmpty = free
+    unsigned* list;   // [slots] occupied slots, listed by the flush
     unsigned* ctr;    // number of occupied slots
-    float* vals;      // [kSlots][16]
+    float* vals;      // [slots][16]
     float* abase;     // gradient base rounded down to 64 bytes (never dereferenced below the base)
     unsigned boff;    // floats from abase to the gradient base
 
-    // every thread of the (one-wave) workgroup; lds is 16-byte aligned, agg_bytes(LOG2) long
-    __device__ __forceinline__ void init(void* lds)
+    __device__ __forceinline__ unsigned slots() const { return 1u << lg; }
+    // every thread of the (one-wave) workgroup; lds is 16-byte aligned, agg_bytes(log2) long
+    __device__ __forceinline__ void init(void* lds, int log2)
     {
+        lg = unsigned(log2);
+        const unsigned kSlots = slots();
         tags = (unsigned*)lds;
         list = tags + kSlots;
         ctr = list + kSlots;
@@ -349,14 +351,14 @@ struct VolAgg {
         abase = (float*)(a & ~63ull);
         boff = unsigned(a & 63ull) >> 2;
     }
-    static __device__ __forceinline__ unsigned home(unsigned q) { return (q * 0x9E3779B1u) >> (32 - LOG2); }
+    __device__ __forceinline__ unsigned home(unsigned q) const { return (q * 0x9E3779B1u) >> (32u - lg); }
     // Resolves a claim whose home slot s returned `old`: linear probing only for
     // the lanes whose slot holds another chunk.  Returns the slot (kAggEmpty: none
     // free within kAggProbe probes) and sets fresh when this call took a free slot.
     __device__ __forceinline__ unsigned settle(unsigned q, unsigned s, unsigned old, bool& fresh)
     {
         for (int p = 1; p < kAggProbe && old != kAggEmpty && old != q; ++p) {
-            s = (s + 1) & (kSlots - 1);
+            s = (s + 1) & (slots() - 1u);
             old = atomicCAS(&tags[s], kAggEmpty, q);
         }
         fresh = old == kAggEmpty;
@@ -423,9 +425,10 @@ struct VolAgg {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         // list the occupied slots (the claims only count them)
         unsigned n = 0u;
-        for (unsigned t0 = 0; t0 < unsigned(kSlots); t0 += cnt) {
+        const unsigned kSlots = slots();
+        for (unsigned t0 = 0; t0 < kSlots; t0 += cnt) {
             const unsigned t = t0 + rank;
-            const bool occ = t < unsigned(kSlots) && tags[t] != kAggEmpty;
+            const bool occ = t < kSlots && tags[t] != kAggEmpty;
             const unsigned long long m = __ballot(occ);
             if (occ) {
                 const unsigned below = __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
@@ -446,7 +449,7 @@ struct VolAgg {
     }
     __device__ __forceinline__ void maybe_flush()
     {
-        if (*ctr > kFlushAt) flush();
+        if (*ctr > (slots() >> 1)) flush();
     }
 };
 
@@ -789,7 +792,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
     const bool aggOn = kVolInst && A.avol != nullptr;
     if constexpr (kVolInst) {
         if (aggOn) {
-            vs.init(lds + agg_offset(RC, ng));
+            vs.init(lds + agg_offset(RC, ng), kAggLog2);
             vs.bind(A.avol + tc.b * A.avSb);
         }
     }
