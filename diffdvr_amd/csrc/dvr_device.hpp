@@ -430,8 +430,9 @@ constexpr int tf_cols()
 // Interval of a piecewise-linear TF that holds a density: rows i and i+1, their
 // control-point positions, and (FAST) the reciprocal of the interval width.
 // FAST instances (the tiled, batched and forward-mode kernels) multiply by
-// v_rcp_f32's reciprocal (1 ulp) instead of dividing; the plain kernels keep
-// the reference's divisions (renderer_tf.cuh:309,337-357).
+// v_rcp_f32's reciprocal (1 ulp) instead of dividing, for the linear and the
+// Gaussian TFs; the plain kernels keep the reference's divisions
+// (renderer_tf.cuh:309,337-357,505-519,580-600).
 struct TfLoc {
     int i;
     float p0, p1, inv;
@@ -448,6 +449,16 @@ __device__ __forceinline__ TfLoc tf_linear_locate(const Tf& T, float density)
     L.p1 = T.p[(i + 1) * T.sr + 4 * T.sc];
     L.inv = FAST ? __builtin_amdgcn_rcpf(L.p1 - L.p0) : 0.0f;
     return L;
+}
+
+// One Gaussian component's weight exp(-(d - mu)^2 / (2 s^2)); FAST multiplies by
+// v_rcp_f32's reciprocal instead of dividing (one expression for the value, the
+// absorption and the adjoint, so the three agree bitwise).
+template <bool FAST>
+__device__ __forceinline__ float gauss_n(float density, float mu, float s)
+{
+    if constexpr (FAST) return expf(-(density - mu) * (density - mu) * __builtin_amdgcn_rcpf(2.0f * s * s));
+    else return expf(-(density - mu) * (density - mu) / (2.0f * s * s));
 }
 
 template <int TFM, bool FAST = false>
@@ -479,7 +490,7 @@ __device__ __forceinline__ f4 tf_eval(const Tf& T, float density, const TfLoc* l
         for (int i = 0; i < T.R; ++i) {
             const float* r = T.p + i * T.sr;
             const float mu = r[4 * T.sc], s = r[5 * T.sc];
-            const float n = expf(-(density - mu) * (density - mu) / (2.0f * s * s));
+            const float n = gauss_n<FAST>(density, mu, s);
             o.x += r[0] * n; o.y += r[T.sc] * n; o.z += r[2 * T.sc] * n; o.w += r[3 * T.sc] * n;
         }
         return o;
@@ -511,7 +522,7 @@ __device__ __forceinline__ float tf_absorption(const Tf& T, float density)
         for (int i = 0; i < T.R; ++i) {
             const float* r = T.p + i * T.sr;
             const float mu = r[4 * T.sc], s = r[5 * T.sc];
-            o += r[3 * T.sc] * expf(-(density - mu) * (density - mu) / (2.0f * s * s));
+            o += r[3 * T.sc] * gauss_n<FAST>(density, mu, s);
         }
         return o;
     }
@@ -743,12 +754,12 @@ __device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4
             const float c0 = r[0], c1 = r[T.sc], c2 = r[2 * T.sc], c3 = r[3 * T.sc];
             const float mu = r[4 * T.sc], s = r[5 * T.sc];
             const float s2 = s * s;
-            const float n = expf(-(density - mu) * (density - mu) / (2.0f * s * s));
-            const float f1 = (mu - density) / s2 * n;
+            const float n = gauss_n<FAST>(density, mu, s);
+            const float f1 = FAST ? (mu - density) * __builtin_amdgcn_rcpf(s2) * n : (mu - density) / s2 * n;
             const float g = adj.x * (c0 * f1) + adj.y * (c1 * f1) + adj.z * (c2 * f1) + adj.w * (c3 * f1);
             adj_d += g;
             if constexpr (WANT_TF) {
-                const float f2 = f1 * (mu - density) / s;
+                const float f2 = FAST ? f1 * (mu - density) * __builtin_amdgcn_rcpf(s) : f1 * (mu - density) / s;
                 sink.add(i, 0, adj.x * n);
                 sink.add(i, 1, adj.y * n);
                 sink.add(i, 2, adj.z * n);

@@ -112,7 +112,7 @@ __device__ __forceinline__ void tf_deriv(const Tf& T, float density, f4& ds, con
         for (int i = 0; i < T.R; ++i) {
             const float* r = T.p + i * C;
             const float mu = r[4], sg = r[5];
-            const float n = expf(-(density - mu) * (density - mu) / (2.0f * sg * sg));
+            const float n = gauss_n<true>(density, mu, sg);  // as the colour (tf_eval<3, true>)
             const float dn = n * (mu - density) / (sg * sg);
             ds.x += r[0] * dn; ds.y += r[1] * dn; ds.z += r[2] * dn; ds.w += r[3] * dn;
 #pragma unroll
