@@ -460,7 +460,7 @@ dvr_batch_plan* create(const dvr_batch* b, const dvr_batch_options* o)
     else {
         if (!anyTfGrad && !anyVolGrad && !anyCamGrad && !p->lossOut)
             fail(DVR_ERR_NO_GRADIENT, "At least one adjoint variable must be activated!");
-        p->gm = anyCamGrad ? 7 : (anyVolGrad ? 3 : 1);
+        p->gm = anyCamGrad ? (anyVolGrad ? 7 : 5) : (anyVolGrad ? 3 : 1);
         p->volGrad = anyVolGrad;
     }
     p->usePriv = p->gm && b->tf_mode == DVR_TF_GAUSSIAN && p->maxRC <= 48;

@@ -276,6 +276,7 @@ static hipError_t batched_fbk(const BatchParams& P, int tfm, int gm, int grid, s
         else return hipErrorInvalidValue;                                          \
     case 1: return batched_k<FILTER, TV, BLEND, K, 1>(P, grid, lds, s, occ);       \
     case 3: return batched_k<FILTER, TV, BLEND, K, 3>(P, grid, lds, s, occ);       \
+    case 5: return batched_k<FILTER, TV, BLEND, K, 5>(P, grid, lds, s, occ);       \
     case 7: return batched_k<FILTER, TV, BLEND, K, 7>(P, grid, lds, s, occ);       \
     default: return hipErrorInvalidValue;                                          \
     }
