@@ -529,7 +529,8 @@ def bench_c4(args):
     loss = torch.zeros(1, device=dev)
     plan = bt.BatchPlan([bt.BatchVolume(vol)], [bt.BatchTf(tf)], views, volume_filter_mode=1, tf_mode=3,
                         blend_mode=0, loss="l2", loss_out=loss,
-                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8)
+                        checkpoint_interval=args.checkpoint_interval or 16, wave_width=args.wave_width or 8,
+                        grid=args.grid)
 
     def step():
         for p in (lat, lon, rad):
@@ -538,6 +539,13 @@ def bench_c4(args):
         vp_buf.copy_(vp.detach())
         plan.run(zero_gradients=True)
         vp.backward(g_vp)
+
+    tuned = None
+    if not (args.checkpoint_interval or args.wave_width or args.grid):
+        with torch.no_grad():  # the plan reads the cameras from vp_buf
+            vp_buf.copy_(pr.Camera.viewport_from_sphere(torch.zeros(N, 3, device=dev), lon, lat, rad,
+                                                        pr.Orientation.Yp))
+        tuned = plan.autotune(iterations=1)
 
     for _ in range(args.warmup):
         step()
@@ -551,7 +559,7 @@ def bench_c4(args):
                           "latitude/longitude/radius, one persistent launch",
               "cameras": N, "rays": int(sum(w * h for w, h in cams.sizes)), "samples_per_step": n_samples,
               **plan.options}
-    result = result_line(args, "c4", value, ms, n_samples, config, k_ms)
+    result = result_line(args, "c4", value, ms, n_samples, config, k_ms, {"autotuned": tuned is not None})
     if not args.profile:
         inp = pr.RendererInputs()
         inp.volume = vol.unsqueeze(0)
