@@ -39,6 +39,7 @@ namespace dvr {
 // it 3 %).
 constexpr int kLdsPerCu = 160 * 1024;  // gfx950
 constexpr int kTiledWgPerCu = 8;
+constexpr int kTiledVolWgPerCu = 16;  // volume + TF gradient instances (four waves per SIMD)
 constexpr int tiled_wg_per_cu(int tfm) { return tfm == 3 ? 0 : kTiledWgPerCu; }
 size_t occupancy_lds(size_t need, int wgPerCu);
 

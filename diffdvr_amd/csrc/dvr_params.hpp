@@ -16,7 +16,8 @@ constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round
 constexpr int kMaxTfLds = 4096;   // max R*C floats of a TF staged in LDS by the tiled kernels
 // LDS table of the volume-gradient aggregation (one per wave): 2^log2 slots of
 // one 64-byte chunk each, plus tags, slot list and counter.  The per-call tiled
-// adjoint uses kAggLog2; a batched plan takes the largest table (kAggMinLog2 ..
+// adjoint uses kAggLog2 where it also carries the position adjoint; its volume +
+// TF instances and a batched plan take the largest table (kAggMinLog2 ..
 // kAggLog2) that keeps its instance at batched_vol_waves waves per SIMD.
 constexpr int kAggLog2 = 8;
 constexpr int kAggMinLog2 = 5;

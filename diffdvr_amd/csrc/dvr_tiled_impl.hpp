@@ -770,10 +770,13 @@ __global__ __launch_bounds__(kBlock) void k_forward_tiled(RenderParams P, CkptPa
     P.term[tc.b * P.termSb + tc.y * P.termSy + tc.x * P.termSx] = n;
 }
 
+// Volume + TF gradient instances (GM 3) are compiled for four waves per SIMD
+// (the walk is latency-bound, as in the batched kernel); the others keep the
+// compiler's default register budget.
 template <int FILTER, int TFM, int BLEND, int K, int GM>
-__global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, AdjointParams A, CkptParams CK, LossParams L,
-                                                          int fused, int ckFromFwd, int usePriv, int tilesX,
-                                                          int tilesPerView, int nTiles)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GM == 3 ? 4 : 1))) void k_adjoint_tiled(
+    RenderParams P, AdjointParams A, CkptParams CK, LossParams L, int fused, int ckFromFwd, int usePriv, int tilesX,
+    int tilesPerView, int nTiles, int aggLog2)
 {
     extern __shared__ float lds[];
     const int RC = P.R * P.C;
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_tiled(RenderParams P, Adjoin
     const bool aggOn = kVolInst && A.avol != nullptr;
     if constexpr (kVolInst) {
         if (aggOn) {
-            vs.init(lds + agg_offset(RC, ng), kAggLog2);
+            vs.init(lds + agg_offset(RC, ng), aggLog2);
             vs.bind(A.avol + tc.b * A.avSb);
         }
     }
@@ -910,11 +913,24 @@ static hipError_t adj_tiled_k(const RenderParams& P, const AdjointParams& A, con
     const int RC = P.R * P.C;
     const bool usePriv = (TFM == 3) && A.atf && RC <= 48;  // private slices <= 48 KiB of LDS
     const int ng = A.atf ? (usePriv ? RC * kBlock : RC) : 0;
-    const size_t need = ((GM & 2) && A.avol) ? sizeof(float) * size_t(agg_offset(RC, ng)) + kAggBytes
-                                             : sizeof(float) * (size_t(RC) + size_t(ng));
-    const size_t lds = occupancy_lds(need, tiled_wg_per_cu(TFM));
+    int aggLog2 = kAggLog2;
+    size_t lds;
+    if ((GM & 2) && A.avol) {
+        const size_t base = sizeof(float) * size_t(agg_offset(RC, ng));
+        if (GM == 3) {
+            // the largest chunk table beside the TF tables that leaves room for sixteen one-wave
+            // workgroups per CU (the four waves per SIMD the instance's registers allow)
+            while (aggLog2 > kAggMinLog2 && base + agg_bytes(aggLog2) > size_t(kLdsPerCu) / kTiledVolWgPerCu)
+                --aggLog2;
+            lds = base + agg_bytes(aggLog2);
+        } else {
+            lds = occupancy_lds(base + agg_bytes(aggLog2), tiled_wg_per_cu(TFM));
+        }
+    } else {
+        lds = occupancy_lds(sizeof(float) * (size_t(RC) + size_t(ng)), tiled_wg_per_cu(TFM));
+    }
     hipLaunchKernelGGL((k_adjoint_tiled<FILTER, TFM, BLEND, K, GM>), dim3(nTiles), dim3(kBlock), lds, s, P, A, CK, L,
-                       fused ? 1 : 0, ckFromFwd ? 1 : 0, usePriv ? 1 : 0, tilesX, tilesX * tilesY, nTiles);
+                       fused ? 1 : 0, ckFromFwd ? 1 : 0, usePriv ? 1 : 0, tilesX, tilesX * tilesY, nTiles, aggLog2);
     return hipGetLastError();
 }
 
