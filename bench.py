@@ -448,7 +448,8 @@ def bench_c3(args):
     value = n_samples / (ms * 1e-3)
     config = {"workload": "C3: 256^3 blob volume + noise 0.02, identity TF x10, 64 Fibonacci views 256x256, "
                           "step 0.5 voxel, L2 vs a second volume, gradient w.r.t. all voxels, K autotuned",
-              "views": B, "image": [H, W], "samples_per_step": n_samples, **plan.options}
+              "views": B, "image": [H, W], "samples_per_step": n_samples, **plan.options,
+              "chunk_slots": plan.chunk_slots}
     # the persistent kernel alone (HIP events around it on the launch stream)
     k_ms = float(np.median([plan.kernel_ms() for _ in range(5)]))
     result = result_line(args, "c3", value, ms, n_samples, config, k_ms,
@@ -558,7 +559,7 @@ def bench_c4(args):
                           "{64,128,256,512}^2, step 0.5 voxel, L2 vs perturbed-camera images, gradient w.r.t. "
                           "latitude/longitude/radius, one persistent launch",
               "cameras": N, "rays": int(sum(w * h for w, h in cams.sizes)), "samples_per_step": n_samples,
-              **plan.options}
+              **plan.options, "chunk_slots": plan.chunk_slots}
     result = result_line(args, "c4", value, ms, n_samples, config, k_ms, {"autotuned": tuned is not None})
     if not args.profile:
         inp = pr.RendererInputs()
@@ -833,7 +834,8 @@ def bench_c5(args):
                           f"{G} steps captured as one HIP graph",
               "volumes": len(vols), "views": len(views), "rays": int(sum(w * h for w, h in sizes)),
               "samples_per_step": n_samples, "samples_first_last": [n0, n1], "graph_steps": G, "graph_replays": reps,
-              "loss_before_graph": loss0, "loss_last_step": loss1, "adam_steps": adam.step_count, **plan.options}
+              "loss_before_graph": loss0, "loss_last_step": loss1, "adam_steps": adam.step_count, **plan.options,
+              "chunk_slots": plan.chunk_slots}
     result = result_line(args, "c5", value, ms, n_samples, config, k_ms, {"autotuned": tuned is not None})
     if plain is not None:
         plain["value"] = n_samples / (plain["ms_per_step"] * 1e-3)

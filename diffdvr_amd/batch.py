@@ -210,6 +210,11 @@ class BatchPlan:
     def workspace_bytes(self) -> int:
         return int(self._h.workspace_bytes())
 
+    @property
+    def chunk_slots(self) -> int:
+        """Slots of the per-wave LDS chunk table of the volume gradients (0: none)."""
+        return int(self._h.chunk_slots())
+
 
 class Adam:
     """torch.optim.Adam (no weight decay, no amsgrad) over several tensors in one HIP launch.

@@ -742,6 +742,12 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
     });
 }
 
+int32_t dvr_batch_plan_chunk_slots(const dvr_batch_plan* plan)
+{
+    if (!plan) return -1;
+    return plan->aggOffset > 0 ? (int32_t(1) << plan->aggLog2) : 0;
+}
+
 int64_t dvr_batch_plan_workspace_bytes(const dvr_batch_plan* plan)
 {
     if (!plan) return -1;

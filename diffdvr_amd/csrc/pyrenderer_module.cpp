@@ -796,7 +796,8 @@ PYBIND11_MODULE(_pyrenderer, m)
             check(dvr_batch_plan_autotune(h.plan, iterations, &o, reinterpret_cast<void*>(stream)));
             return py::make_tuple(o.checkpoint_interval, o.wave_width, o.grid, o.reinterpolate != 0);
         })
-        .def("workspace_bytes", [](const BatchPlanHandle& h) { return dvr_batch_plan_workspace_bytes(h.plan); });
+        .def("workspace_bytes", [](const BatchPlanHandle& h) { return dvr_batch_plan_workspace_bytes(h.plan); })
+        .def("chunk_slots", [](const BatchPlanHandle& h) { return dvr_batch_plan_chunk_slots(h.plan); });
 
     py::class_<AdamHandle>(m, "_AdamGroup")
         .def(py::init([](py::list blocks, int device) {

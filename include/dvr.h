@@ -469,6 +469,12 @@ dvr_status dvr_batch_plan_set_options(dvr_batch_plan* plan, const dvr_batch_opti
  * off).  A cached result is applied without timing runs.  Timing runs
  * overwrite the outputs; the gradients are left ZEROED either way. */
 dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr_batch_options* chosen, void* stream);
+/* Slots of the per-wave LDS table that sums the volume-gradient corners per 64-byte chunk
+ * before the device atomics (the largest of 32..256 that keeps the volume-gradient instance at
+ * four waves per SIMD beside the plan's TF tables, for the current options); 0 when the plan
+ * takes no volume gradients, -1 for NULL.  No counterpart in the reference (it scatters every
+ * corner with a device atomic, renderer_interpolation.cuh:299-324). */
+int32_t dvr_batch_plan_chunk_slots(const dvr_batch_plan* plan);
 /* Bytes of device scratch the plan holds (checkpoints, tables). */
 int64_t dvr_batch_plan_workspace_bytes(const dvr_batch_plan* plan);
 

@@ -59,12 +59,13 @@ def make_tf(rng, tf_mode, R=8, spaced=True):
 class Case:
     """A heterogeneous batch as numpy (oracle) and device tensors (plan)."""
 
-    def __init__(self, seed, filter, tf_mode, blend, grads=("tf", "volume", "camera"), loss=None, spaced=True):
+    def __init__(self, seed, filter, tf_mode, blend, grads=("tf", "volume", "camera"), loss=None, spaced=True,
+                 tf_rows=8):
         rng = np.random.default_rng(seed)
         self.filter, self.tf_mode, self.blend = filter, tf_mode, blend
         self.vols = [to_f32(scenes.blob_volume(sh, 6, int(rng.integers(1 << 30)))[0].double().numpy())
                      for sh in VOL_SHAPES]
-        self.tfs = [make_tf(rng, tf_mode, spaced=spaced) for _ in range(2)]
+        self.tfs = [make_tf(rng, tf_mode, R=tf_rows, spaced=spaced) for _ in range(2)]
         if blend == 1:
             for t in self.tfs:
                 t[..., 0 if tf_mode == 0 else 3] *= 0.5
@@ -128,6 +129,67 @@ def oracle_batch(case, terms, adj_colors):
     return colors, g_vol, g_tf, g_cam
 
 
+def check_against_oracle(case, twin_bound=False):
+    """Forward, termination and every requested gradient of a run plan against the per-view oracle.
+    twin_bound: the forward bound of a view is max(1e-5, 3x the FP32 CPU twin's error) (for TFs whose
+    slope amplifies FP32 density rounding beyond 1e-5 in any FP32 evaluation)."""
+    filter = case.filter
+    terms = [v.termination.cpu().numpy() for v in case.views]
+    colors, g_vol, g_tf, g_cam = oracle_batch(case, terms, case.adj_colors)
+    # samples within FP32 rounding of a kink of the reference's derivative (TF clamp /
+    # texel edge / control point, alpha clamp): FP32 may take the other side there
+    census = [oracle.kink_census(case.scene(i), terms[i][None])[0] for i in range(len(VIEWS))]
+    kinks = [int(c.sum()) for c in census]
+    print("kink samples per view", kinks)
+    for i, v in enumerate(case.views):
+        c_free, t_free = oracle.forward(case.scene(i))
+        # a ray may stop one sample apart where its transmittance ties with the early-out threshold
+        # within FP32 rounding (the window of the smooth mode matrix: the transmittance form carries
+        # up to ~1e-3 relative error near a threshold of 1e-5 after near-opaque samples); any other
+        # disagreement is limited to 0.1% of the view's rays
+        thr, win = 1e-5, 5e-3 * 1e-5
+        tie = (np.abs((1 - c_free[0][..., 3]) - thr) <= win) | (np.abs((1 - colors[i][..., 3]) - thr) <= win)
+        off = t_free[0] != terms[i]
+        bad = off & ~tie
+        assert np.abs(t_free[0] - terms[i]).max() <= 1 and bad.sum() <= 1e-3 * terms[i].size, \
+            (i, int(off.sum()), int(bad.sum()), [float(1 - c_free[0][tuple(j)][3]) for j in np.argwhere(off)[:4]])
+        c = v.color.double().cpu().numpy()
+        scale = max(np.abs(colors[i]).max(), 1e-30)
+        clean = census[i] == 0  # rays without a sample on a (nearest-filter) value discontinuity
+        err = np.abs(c - colors[i])[clean].max() / scale
+        err_all = np.abs(c - colors[i]).max() / scale
+        bound = 1e-5
+        if twin_bound:
+            c32, _ = oracle.forward(case.scene(i), dtype=np.float32, frozen_term=terms[i][None])
+            twin = np.abs(c32[0].astype(np.float64) - colors[i])[clean].max() / scale
+            print(f"view {i}: forward err {err:.2e}, FP32 twin {twin:.2e}")
+            bound = max(bound, 3.0 * twin)
+        assert err <= bound and err_all <= 1e-3, ("forward", i, err, err_all, bound, int((~clean).sum()))
+    errs = {}
+    for k, (got, want) in enumerate(zip(case.g_tfs, g_tf)):
+        if got is not None:
+            errs[f"tf{k}"] = norm_err(got.double().cpu().numpy(), want)
+    for k, (got, want) in enumerate(zip(case.g_vols, g_vol)):
+        if got is not None:
+            errs[f"vol{k}"] = norm_err(got.double().cpu().numpy()[2:-2, 2:-2, 2:-2], want[2:-2, 2:-2, 2:-2])
+    for k, (got, want) in enumerate(zip(case.g_cams, g_cam)):
+        if got is None:
+            continue
+        g = got.double().cpu().numpy()
+        if filter == 0:  # nearest: no position gradient (renderer_interpolation.cuh:95-106)
+            assert np.abs(g).max() == 0.0 and np.abs(want).max() == 0.0
+        else:
+            errs[f"cam{k}"] = norm_err(g, want)
+    print({k: f"{v:.1e}" for k, v in errs.items()})
+    assert errs or filter == 0
+    views_of = {f"tf{k}": [i for i, w in enumerate(VIEWS) if w[1] == k] for k in range(2)}
+    views_of.update({f"vol{k}": [i for i, w in enumerate(VIEWS) if w[0] == k] for k in range(2)})
+    views_of.update({f"cam{i}": [i] for i in range(len(VIEWS))})
+    for k, e in errs.items():
+        tol = 1e-4 if all(kinks[i] == 0 for i in views_of[k]) else 1e-3
+        assert e <= tol, (k, e, tol)
+
+
 @pytest.mark.parametrize("filter", [0, 1])
 @pytest.mark.parametrize("tf_mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("blend", [0, 1])
@@ -138,41 +200,37 @@ def test_heterogeneous_batch_matches_oracle(filter, tf_mode, blend):
     plan = case.plan()
     plan.run()
     torch.cuda.synchronize()
-    terms = [v.termination.cpu().numpy() for v in case.views]
-    colors, g_vol, g_tf, g_cam = oracle_batch(case, terms, case.adj_colors)
-    # samples within FP32 rounding of a kink of the reference's derivative (TF clamp /
-    # texel edge / control point, alpha clamp): FP32 may take the other side there
-    census = [oracle.kink_census(case.scene(i), terms[i][None])[0] for i in range(len(VIEWS))]
-    kinks = [int(c.sum()) for c in census]
-    print("kink samples per view", kinks)
-    for i, v in enumerate(case.views):
-        _, t_free = oracle.forward(case.scene(i))
-        match = float((t_free[0] == terms[i]).mean())
-        assert match >= 0.999 and np.abs(t_free[0] - terms[i]).max() <= 1, (i, match)
-        c = v.color.double().cpu().numpy()
-        scale = max(np.abs(colors[i]).max(), 1e-30)
-        clean = census[i] == 0  # rays without a sample on a (nearest-filter) value discontinuity
-        err = np.abs(c - colors[i])[clean].max() / scale
-        err_all = np.abs(c - colors[i]).max() / scale
-        assert err <= 1e-5 and err_all <= 1e-3, ("forward", i, err, err_all, int((~clean).sum()))
-    errs = {}
-    for k, (got, want) in enumerate(zip(case.g_tfs, g_tf)):
-        errs[f"tf{k}"] = norm_err(got.double().cpu().numpy(), want)
-    for k, (got, want) in enumerate(zip(case.g_vols, g_vol)):
-        errs[f"vol{k}"] = norm_err(got.double().cpu().numpy()[2:-2, 2:-2, 2:-2], want[2:-2, 2:-2, 2:-2])
-    for k, (got, want) in enumerate(zip(case.g_cams, g_cam)):
-        g = got.double().cpu().numpy()
-        if filter == 0:  # nearest: no position gradient (renderer_interpolation.cuh:95-106)
-            assert np.abs(g).max() == 0.0 and np.abs(want).max() == 0.0
-        else:
-            errs[f"cam{k}"] = norm_err(g, want)
-    print({k: f"{v:.1e}" for k, v in errs.items()})
-    views_of = {f"tf{k}": [i for i, w in enumerate(VIEWS) if w[1] == k] for k in range(2)}
-    views_of.update({f"vol{k}": [i for i, w in enumerate(VIEWS) if w[0] == k] for k in range(2)})
-    views_of.update({f"cam{i}": [i] for i in range(len(VIEWS))})
-    for k, e in errs.items():
-        tol = 1e-4 if all(kinks[i] == 0 for i in views_of[k]) else 1e-3
-        assert e <= tol, (k, e, tol)
+    check_against_oracle(case)
+
+
+@pytest.mark.parametrize("tf_mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("grads", [("tf", "volume"), ("tf", "camera"), ("camera",)])
+def test_gradient_subsets_match_oracle(tf_mode, grads):
+    """Plans with volume but no camera gradients run the volume instance (four waves per SIMD, chunk
+    table sized beside the TF tables); plans with camera but no volume gradients run the TF + position
+    instance.  Each is held to the oracle like the full-gradient instance."""
+    seed = 7300 + 10 * tf_mode + len(grads)
+    print(f"case tf_mode={tf_mode} grads={grads} seed={seed}", flush=True)
+    case = Case(seed, 1, tf_mode, 0, grads=grads)
+    plan = case.plan()
+    print("chunk table slots", plan.chunk_slots)
+    assert (plan.chunk_slots >= 32) if "volume" in grads else (plan.chunk_slots == 0)
+    plan.run()
+    torch.cuda.synchronize()
+    check_against_oracle(case)
+
+
+def test_large_tf_takes_the_smallest_chunk_table():
+    """A 256-row texture TF and its gradient copies fill most of the LDS a volume-gradient workgroup
+    may hold: the plan falls back to the smallest chunk table (32 slots, flushed most often).  The
+    256-texel TF's slope amplifies FP32 density rounding 4x more than C2's 64 texels: the forward is
+    held to 3x the FP32 CPU twin's error where that exceeds 1e-5."""
+    case = Case(7400, 1, 1, 0, grads=("tf", "volume"), tf_rows=256)
+    plan = case.plan(checkpoint_interval=8)
+    assert plan.chunk_slots == 32, plan.chunk_slots
+    plan.run()
+    torch.cuda.synchronize()
+    check_against_oracle(case, twin_bound=True)
 
 
 def test_ill_conditioned_linear_tf_as_accurate_as_fp32_twin():
