@@ -15,7 +15,7 @@ constexpr int kTfCopies = 8;      // batched kernel: copies of the LDS TF-gradie
 constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kMaxTfLds = 4096;   // max R*C floats of a TF staged in LDS by the tiled kernels
 // LDS table of the volume-gradient aggregation (one per wave): 2^log2 slots of
-// one 64-byte chunk each, plus tags, slot list and counter.  The per-call tiled
+// one 64-byte chunk each, plus tags, slot list and 16 bytes of padding.  The per-call tiled
 // adjoint uses kAggLog2 where it also carries the position adjoint; its volume +
 // TF instances and a batched plan take the largest table (kAggMinLog2 ..
 // kAggLog2) that keeps its instance at batched_vol_waves waves per SIMD.

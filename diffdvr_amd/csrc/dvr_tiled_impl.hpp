@@ -222,9 +222,9 @@ __device__ __forceinline__ unsigned corner_off(const CellOffsets& o, int k)
 // values per distinct 64-byte chunk, so the wave's corner gradients are first
 // summed in an LDS table of chunks (open addressing on the chunk index:
 // ds_cmpst claims a slot, ds_add_f32 adds the value) and the table is flushed
-// with one 16-lane atomic group per occupied chunk.  A claim only counts the
-// slots taken (one LDS add per call, no return), and the flush lists the
-// occupied slots by scanning the tags.  A chunk that finds no free
+// with one 16-lane atomic group per occupied chunk.  The walk checks every four
+// samples how many slots are taken, and the flush lists the occupied slots, by
+// scanning the tags.  A chunk that finds no free
 // slot within kAggProbe probes goes straight to device memory.  The table
 // belongs to one wave (one-wave workgroups) and to one gradient tensor: it is
 // flushed when it is half full, at the end of every tile and before its
@@ -325,7 +325,6 @@ struct VolAgg {
     Layout L;         // element offsets of the gradient the table adds into
     unsigned* tags;   // [slots] chunk index, kAggEmpty = free
     unsigned* list;   // [slots] occupied slots, listed by the flush
-    unsigned* ctr;    // number of occupied slots
     float* vals;      // [slots][16]
     float* abase;     // gradient base rounded down to 64 bytes (never dereferenced below the base)
     unsigned boff;    // floats from abase to the gradient base
@@ -338,11 +337,9 @@ struct VolAgg {
         const unsigned kSlots = slots();
         tags = (unsigned*)lds;
         list = tags + kSlots;
-        ctr = list + kSlots;
-        vals = (float*)(ctr + 4);
+        vals = (float*)(list + kSlots + 4);  // 16 bytes of padding after the list
         for (int e = threadIdx.x; e < kSlots; e += blockDim.x) tags[e] = kAggEmpty;
         for (int e = threadIdx.x; e < kSlots * 16; e += blockDim.x) vals[e] = 0.f;
-        if (threadIdx.x == 0) *ctr = 0u;
     }
     // the table must be empty (flushed) when the base changes
     __device__ __forceinline__ void bind(float* g)
@@ -352,32 +349,18 @@ struct VolAgg {
         boff = unsigned(a & 63ull) >> 2;
     }
     __device__ __forceinline__ unsigned home(unsigned q) const { return (q * 0x9E3779B1u) >> (32u - lg); }
-    // Resolves a claim whose home slot s returned `old`: linear probing only for
-    // the lanes whose slot holds another chunk.  Returns the slot (kAggEmpty: none
-    // free within kAggProbe probes) and sets fresh when this call took a free slot.
-    __device__ __forceinline__ unsigned settle(unsigned q, unsigned s, unsigned old, bool& fresh)
+    // The slot of chunk q: its home slot, or (linear probing, only for the lanes
+    // whose slot holds another chunk) the next free or matching one; kAggEmpty when
+    // none is within kAggProbe probes.
+    __device__ __forceinline__ unsigned claim(unsigned q)
     {
+        unsigned s = home(q);
+        unsigned old = atomicCAS(&tags[s], kAggEmpty, q);
         for (int p = 1; p < kAggProbe && old != kAggEmpty && old != q; ++p) {
             s = (s + 1) & (slots() - 1u);
             old = atomicCAS(&tags[s], kAggEmpty, q);
         }
-        fresh = old == kAggEmpty;
         return (old == kAggEmpty || old == q) ? s : kAggEmpty;
-    }
-    // occupied-slot count: c (uniform over the active lanes) added by the first
-    // active lane, without a return value
-    __device__ __forceinline__ void count(unsigned c)
-    {
-        const unsigned long long act = __ballot(1);
-        if (c != 0u && __lane_id() == unsigned(__ffsll((long long)act) - 1)) atomicAdd(ctr, c);
-    }
-    __device__ __forceinline__ unsigned claim(unsigned q)
-    {
-        const unsigned s0 = home(q);
-        bool fresh;
-        const unsigned s = settle(q, s0, atomicCAS(&tags[s0], kAggEmpty, q), fresh);
-        count(unsigned(__popcll(__ballot(fresh))));
-        return s;
     }
     __device__ __forceinline__ void put(unsigned s, unsigned e, float v)
     {
@@ -423,7 +406,7 @@ struct VolAgg {
         const unsigned cnt = unsigned(__popcll(act));
         const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(act >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(act), 0u));
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        // list the occupied slots (the claims only count them)
+        // list the occupied slots
         unsigned n = 0u;
         const unsigned kSlots = slots();
         for (unsigned t0 = 0; t0 < kSlots; t0 += cnt) {
@@ -444,12 +427,22 @@ struct VolAgg {
             vals[s * 16 + j] = 0.f;
         }
         for (unsigned e = rank; e < n; e += cnt) tags[list[e]] = kAggEmpty;
-        if (rank == 0) *ctr = 0u;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
+    // flush when more than half of the slots are taken: the active lanes count the tags
     __device__ __forceinline__ void maybe_flush()
     {
-        if (*ctr > (slots() >> 1)) flush();
+        const unsigned long long act = __ballot(1);
+        const unsigned cnt = unsigned(__popcll(act));
+        const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(act >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(act), 0u));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const unsigned kSlots = slots();
+        unsigned n = 0u;
+        for (unsigned t0 = 0; t0 < kSlots; t0 += cnt) {
+            const unsigned t = t0 + rank;
+            n += unsigned(__popcll(__ballot(t < kSlots && tags[t] != kAggEmpty)));
+        }
+        if (n > (kSlots >> 1)) flush();
     }
 };
 
