@@ -349,17 +349,27 @@ struct VolAgg {
         boff = unsigned(a & 63ull) >> 2;
     }
     __device__ __forceinline__ unsigned home(unsigned q) const { return (q * 0x9E3779B1u) >> (32u - lg); }
-    // The slot of chunk q: its home slot, or (linear probing, only for the lanes
-    // whose slot holds another chunk) the next free or matching one; kAggEmpty when
-    // none is within kAggProbe probes.
+    // Linear probing, only for the lanes whose slot holds another chunk: nested
+    // rather than a loop, so each level costs one exec-mask save and restore and
+    // the levels below are skipped as soon as no lane needs them.
+    template <int P>
+    __device__ __forceinline__ void probe(unsigned q, unsigned& s, unsigned& old)
+    {
+        if constexpr (P < kAggProbe) {
+            if (old != kAggEmpty && old != q) {
+                s = (s + 1) & (slots() - 1u);
+                old = atomicCAS(&tags[s], kAggEmpty, q);
+                probe<P + 1>(q, s, old);
+            }
+        }
+    }
+    // The slot of chunk q: its home slot or the next free or matching one within
+    // kAggProbe probes; kAggEmpty when there is none.
     __device__ __forceinline__ unsigned claim(unsigned q)
     {
         unsigned s = home(q);
         unsigned old = atomicCAS(&tags[s], kAggEmpty, q);
-        for (int p = 1; p < kAggProbe && old != kAggEmpty && old != q; ++p) {
-            s = (s + 1) & (slots() - 1u);
-            old = atomicCAS(&tags[s], kAggEmpty, q);
-        }
+        probe<1>(q, s, old);
         return (old == kAggEmpty || old == q) ? s : kAggEmpty;
     }
     __device__ __forceinline__ void put(unsigned s, unsigned e, float v)
