@@ -293,7 +293,7 @@ bool load_tuned(const std::string& key, Tuned& out)
     int K, ww, grid;
     bool found = false;
     while (std::fscanf(fp, "%llx %d %d %d", &h, &K, &ww, &grid) == 4)
-        if (h == want && (K == 8 || K == 16 || K == 32 || K == 64) && wave_shift_of(ww) >= 0 && grid > 0) {
+        if (h == want && (K == 4 || K == 8 || K == 16 || K == 32 || K == 64) && wave_shift_of(ww) >= 0 && grid > 0) {
             out = Tuned{K, ww, grid};
             found = true;
         }
@@ -317,8 +317,8 @@ void check_options(const dvr_batch_options* o, int& K, int& ww)
 {
     K = o && o->checkpoint_interval ? o->checkpoint_interval : 16;
     ww = o && o->wave_width ? o->wave_width : 8;
-    if (K != 8 && K != 16 && K != 32 && K != 64)
-        fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 8, 16, 32 or 64");
+    if (K != 4 && K != 8 && K != 16 && K != 32 && K != 64)
+        fail(DVR_ERR_INVALID_ARGUMENT, "checkpoint_interval must be 4, 8, 16, 32 or 64");
     if (wave_shift_of(ww) < 0) fail(DVR_ERR_INVALID_ARGUMENT, "wave_width must be 4, 8, 16 or 32");
     if (o && o->grid < 0) fail(DVR_ERR_INVALID_ARGUMENT, "grid must be >= 0");
 }
@@ -693,7 +693,7 @@ dvr_status dvr_batch_plan_autotune(dvr_batch_plan* plan, int32_t iterations, dvr
                 Tuned cur{16, 8, 0};
                 float bestMs = 1e30f;
                 std::vector<int> Ks;
-                if (plan->gm) Ks = {8, 16, 32, 64};
+                if (plan->gm) Ks = {4, 8, 16, 32, 64};
                 else Ks = {16};
                 for (int K : Ks) {
                     {

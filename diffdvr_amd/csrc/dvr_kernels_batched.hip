@@ -6,7 +6,7 @@ namespace dvr {
 
 #define DVR_DECL_INST(F, B, K)                                                                                  \
     hipError_t batched_inst_##F##_##B##_##K(const BatchParams&, int, int, int, size_t, hipStream_t, int*);
-#define DVR_DECL_K(F, B) DVR_DECL_INST(F, B, 8) DVR_DECL_INST(F, B, 16) DVR_DECL_INST(F, B, 32)
+#define DVR_DECL_K(F, B) DVR_DECL_INST(F, B, 4) DVR_DECL_INST(F, B, 8) DVR_DECL_INST(F, B, 16) DVR_DECL_INST(F, B, 32)
 DVR_DECL_K(0, 0)
 DVR_DECL_K(0, 1)
 DVR_DECL_K(1, 0)
@@ -22,6 +22,7 @@ hipError_t launch_batched(const BatchParams& P, int filter, int tfm, int blend, 
 #define DVR_CALL(F, B, KK) return batched_inst_##F##_##B##_##KK(P, tfm, gm, grid, lds, s, occ)
 #define DVR_SEL_K(F, B)                      \
     switch (K) {                             \
+    case 4: DVR_CALL(F, B, 4);               \
     case 8: DVR_CALL(F, B, 8);               \
     case 16: DVR_CALL(F, B, 16);             \
     case 32: DVR_CALL(F, B, 32);             \

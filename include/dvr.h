@@ -427,7 +427,7 @@ typedef struct dvr_batch {
 } dvr_batch;
 
 typedef struct dvr_batch_options {
-    int32_t checkpoint_interval; /* K in {8,16,32,64}; 0 = 16 */
+    int32_t checkpoint_interval; /* K in {4,8,16,32,64}; 0 = 16 */
     int32_t wave_width;          /* rays along x of a wave's tile: 4, 8, 16 or 32 (tile 4x16 .. 32x2); 0 = 8 */
     int32_t grid;                /* persistent workgroups; 0 = 2 per compute unit (within residency) */
     int32_t reinterpolate;       /* 0: the forward march stores every sample's density and the reverse march
@@ -457,7 +457,7 @@ dvr_status dvr_batch_plan_get_options(const dvr_batch_plan* plan, dvr_batch_opti
 /* Re-layout for new options; may allocate (set-up only, not under capture). */
 dvr_status dvr_batch_plan_set_options(dvr_batch_plan* plan, const dvr_batch_options* options);
 /* Autotuner: times candidates on this plan's own batch (`iterations` timed
- * runs each, HIP events) in two stages -- checkpoint interval {8,16,32,64}
+ * runs each, HIP events) in two stages -- checkpoint interval {4,8,16,32,64}
  * jointly with workgroups per CU (1..4 waves per SIMD and full residency, per
  * interval, unless options.grid was set), then wave tile width {8,16,4,32} --
  * keeps the fastest and writes it to *chosen.  Results are

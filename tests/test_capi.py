@@ -181,7 +181,7 @@ def _plan(volumes=None, tfs=None, views=None, tf_mode=1, K=16, ww=8, loss=0, for
     (dict(views=[(0, 0, 8, 6, 2, 0x5000, 0.7, 0.01, 0x6000, 0x7000, 0, 0, 0.0, 0)]), "adj_color is NULL"),
     (dict(loss=1), "target is NULL"),
     (dict(volumes=[(0x1000, 1, 5, 5, [-0.5] * 3, [1.0] * 3, 0)]), "at least 2 voxels"),
-    (dict(K=12), "checkpoint_interval must be 8, 16, 32 or 64"),
+    (dict(K=12), "checkpoint_interval must be 4, 8, 16, 32 or 64"),
     (dict(ww=6), "wave_width must be 4, 8, 16 or 32"),
     (dict(tf_mode=6), "tf mode 6 is not supported"),
     (dict(device=-1), "GPU only"),

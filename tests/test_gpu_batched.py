@@ -331,7 +331,7 @@ def test_autotune_keeps_results():
     ref_g = [g.clone() for g in case.g_tfs + case.g_vols + case.g_cams]
     chosen = plan.autotune(iterations=2)
     print("autotuned", chosen)
-    assert chosen["checkpoint_interval"] in (8, 16, 32, 64) and chosen["wave_width"] in (4, 8, 16, 32)
+    assert chosen["checkpoint_interval"] in (4, 8, 16, 32, 64) and chosen["wave_width"] in (4, 8, 16, 32)
     assert all(float(g.abs().max()) == 0.0 for g in case.g_tfs + case.g_vols + case.g_cams)  # left zeroed
     plan.run()
     torch.cuda.synchronize()
@@ -341,19 +341,21 @@ def test_autotune_keeps_results():
         assert norm_err(a.double().cpu().numpy(), b.double().cpu().numpy()) <= 1e-5
 
 
-def test_every_option_gives_the_same_gradients():
-    case = Case(7600, 1, 2, 1)
+@pytest.mark.parametrize("grads", [("tf", "volume", "camera"), ("tf", "volume")])
+def test_every_option_gives_the_same_gradients(grads):
+    case = Case(7600, 1, 2, 1, grads=grads)
     plan = case.plan()
     plan.run()
     torch.cuda.synchronize()
-    ref = [g.clone() for g in case.g_tfs + case.g_vols + case.g_cams]
-    for K in (8, 16, 32, 64):
+    got = lambda: [g for g in case.g_tfs + case.g_vols + case.g_cams if g is not None]
+    ref = [g.clone() for g in got()]
+    for K in (4, 8, 16, 32, 64):
         for ww in (4, 8, 16, 32):
             for grid in (0, 8, 37):
                 plan.set_options(K, ww, grid, density_store=(grid != 8))
                 plan.run()
                 torch.cuda.synchronize()
-                for a, b in zip(ref, case.g_tfs + case.g_vols + case.g_cams):
+                for a, b in zip(ref, got()):
                     e = norm_err(a.double().cpu().numpy(), b.double().cpu().numpy())
                     assert e <= 1e-5, (K, ww, grid, e)
 
