@@ -1171,8 +1171,8 @@ def bench_preshade(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: only --gpus 1 is supported; this build measures one MI355X "
-              f"(multi-GPU sharding is a design note, not a build target)", file=sys.stderr)
+        print(f"bench.py: only --gpus 1 is supported; the benchmark measures one MI355X "
+              f"(sharding the views over several GPUs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
     fn = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "fwd": bench_fwd,
           "preshade": bench_preshade}[args.workload]
