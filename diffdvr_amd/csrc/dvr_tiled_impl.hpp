@@ -937,11 +937,13 @@ static hipError_t adj_tiled_k(const RenderParams& P, const AdjointParams& A, con
     return hipGetLastError();
 }
 
-// gradient kinds compiled into the instance: TF only, volume (+TF), or everything
+// gradient kinds compiled into the instance: TF only, volume (+TF), position (+TF), or everything
+// (camera / step gradients without volume gradients run an instance compiled
+// without the volume-gradient code: fewer registers)
 inline int tiled_grad_mask(const AdjointParams& A)
 {
     const bool pos = A.ars || A.ast;
-    return pos ? 7 : (A.avol ? 3 : 1);
+    return pos ? (A.avol ? 7 : 5) : (A.avol ? 3 : 1);
 }
 
 template <int FILTER, int BLEND, int K>
@@ -952,6 +954,7 @@ static hipError_t adj_tiled_fbk(const RenderParams& P, const AdjointParams& A, c
 #define DVR_GM_CASES(TV)                                                                              \
     if (gm == 1) return adj_tiled_k<FILTER, TV, BLEND, K, 1>(P, A, CK, L, fused, ckFromFwd, s);      \
     if (gm == 3) return adj_tiled_k<FILTER, TV, BLEND, K, 3>(P, A, CK, L, fused, ckFromFwd, s);      \
+    if (gm == 5) return adj_tiled_k<FILTER, TV, BLEND, K, 5>(P, A, CK, L, fused, ckFromFwd, s);      \
     return adj_tiled_k<FILTER, TV, BLEND, K, 7>(P, A, CK, L, fused, ckFromFwd, s);
     switch (tfm) {
     case 0: { DVR_GM_CASES(0) }
