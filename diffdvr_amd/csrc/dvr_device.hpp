@@ -461,8 +461,27 @@ __device__ __forceinline__ float gauss_n(float density, float mu, float s)
     else return expf(-(density - mu) * (density - mu) / (2.0f * s * s));
 }
 
+// A sample's TF evaluation shared by its value and its adjoint in the reverse
+// walk: the linear TF's located interval, or the first kGaussCache Gaussian
+// weights (each an exponential, otherwise evaluated twice per sample).
+constexpr int kGaussCache = 8;
+struct TfPre {
+    TfLoc L;
+    float gn[kGaussCache];
+};
+template <int TFM, bool FAST>
+__device__ __forceinline__ void tf_prepare(const Tf& T, float density, TfPre& pre)
+{
+    if constexpr (TFM == 2) pre.L = tf_linear_locate<FAST>(T, density);
+    if constexpr (TFM == 3) {
+#pragma unroll
+        for (int i = 0; i < kGaussCache; ++i)
+            pre.gn[i] = i < T.R ? gauss_n<FAST>(density, T.p[i * T.sr + 4 * T.sc], T.p[i * T.sr + 5 * T.sc]) : 0.0f;
+    }
+}
+
 template <int TFM, bool FAST = false>
-__device__ __forceinline__ f4 tf_eval(const Tf& T, float density, const TfLoc* loc = nullptr)
+__device__ __forceinline__ f4 tf_eval(const Tf& T, float density, const TfPre* pre = nullptr)
 {
     if constexpr (TFM == 0) {  // identity (renderer_tf.cuh:62-73)
         const float d = fclamp(density, 0.0f, 1.0f);
@@ -477,7 +496,7 @@ __device__ __forceinline__ f4 tf_eval(const Tf& T, float density, const TfLoc* l
         return mk4(lerpf_(a[0], c[0], df), lerpf_(a[T.sc], c[T.sc], df), lerpf_(a[2 * T.sc], c[2 * T.sc], df),
                    lerpf_(a[3 * T.sc], c[3 * T.sc], df));
     } else if constexpr (TFM == 2) {  // piecewise linear (:294-313)
-        const TfLoc L = loc ? *loc : tf_linear_locate<FAST>(T, density);
+        const TfLoc L = pre ? pre->L : tf_linear_locate<FAST>(T, density);
         const float* a = T.p + L.i * T.sr;
         const float* c = a + T.sr;
         const float p0 = L.p0, p1 = L.p1;
@@ -487,11 +506,20 @@ __device__ __forceinline__ f4 tf_eval(const Tf& T, float density, const TfLoc* l
                    lerpf_(a[3 * T.sc], c[3 * T.sc], f));
     } else {  // sum of Gaussians, column 5 = sigma (:491-519)
         f4 o = mk4(0.f, 0.f, 0.f, 0.f);
-        for (int i = 0; i < T.R; ++i) {
+        auto comp = [&](int i, float n) __attribute__((always_inline)) {
             const float* r = T.p + i * T.sr;
-            const float mu = r[4 * T.sc], s = r[5 * T.sc];
-            const float n = gauss_n<FAST>(density, mu, s);
             o.x += r[0] * n; o.y += r[T.sc] * n; o.z += r[2 * T.sc] * n; o.w += r[3 * T.sc] * n;
+        };
+        int i0 = 0;
+        if (pre) {
+#pragma unroll
+            for (int i = 0; i < kGaussCache; ++i)
+                if (i < T.R) comp(i, pre->gn[i]);
+            i0 = kGaussCache;
+        }
+        for (int i = i0; i < T.R; ++i) {
+            const float* r = T.p + i * T.sr;
+            comp(i, gauss_n<FAST>(density, r[4 * T.sc], r[5 * T.sc]));
         }
         return o;
     }
@@ -701,7 +729,7 @@ __device__ __forceinline__ f4 blend_invert(const f4& acc_out, const f4& s, float
 // ---------------------------------------------------------------------------
 template <int TFM, bool WANT_TF, class Sink, bool FAST = false>
 __device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4& adj, Sink& sink,
-                                            const TfLoc* loc = nullptr)
+                                            const TfPre* pre = nullptr)
 {
     if constexpr (TFM == 0) {
         const float dc = fclamp(density, 0.0f, 1.0f);
@@ -724,7 +752,7 @@ __device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4
         if constexpr (WANT_TF) sink.template add_pair<false>(i0, 1.0f - df, i1, df, adj, 0.0f, 0.0f);
         return adj_df * float(T.R);
     } else if constexpr (TFM == 2) {
-        const TfLoc L = loc ? *loc : tf_linear_locate<FAST>(T, density);
+        const TfLoc L = pre ? pre->L : tf_linear_locate<FAST>(T, density);
         const int i = L.i;
         const float* a = T.p + i * T.sr;
         const float* c = a + T.sr;
@@ -749,12 +777,11 @@ __device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4
         return FAST ? adj_f * L.inv : adj_f / (p1 - p0);  // passes through the clamp (:350-351)
     } else {
         float adj_d = 0.0f;
-        for (int i = 0; i < T.R; ++i) {
+        auto comp = [&](int i, float n) __attribute__((always_inline)) {
             const float* r = T.p + i * T.sr;
             const float c0 = r[0], c1 = r[T.sc], c2 = r[2 * T.sc], c3 = r[3 * T.sc];
             const float mu = r[4 * T.sc], s = r[5 * T.sc];
             const float s2 = s * s;
-            const float n = gauss_n<FAST>(density, mu, s);
             const float f1 = FAST ? (mu - density) * __builtin_amdgcn_rcpf(s2) * n : (mu - density) / s2 * n;
             const float g = adj.x * (c0 * f1) + adj.y * (c1 * f1) + adj.z * (c2 * f1) + adj.w * (c3 * f1);
             adj_d += g;
@@ -767,6 +794,17 @@ __device__ __forceinline__ float tf_adjoint(const Tf& T, float density, const f4
                 sink.add(i, 4, -g);
                 sink.add(i, 5, adj.x * (c0 * f2) + adj.y * (c1 * f2) + adj.z * (c2 * f2) + adj.w * (c3 * f2));
             }
+        };
+        int i0 = 0;
+        if (pre) {
+#pragma unroll
+            for (int i = 0; i < kGaussCache; ++i)
+                if (i < T.R) comp(i, pre->gn[i]);
+            i0 = kGaussCache;
+        }
+        for (int i = i0; i < T.R; ++i) {
+            const float* r = T.p + i * T.sr;
+            comp(i, gauss_n<FAST>(density, r[4 * T.sc], r[5 * T.sc]));
         }
         return adj_d;
     }

@@ -655,19 +655,19 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
         auto walk = [&](int j) __attribute__((always_inline)) {
             const int i = i0 + j;
             const float t = r.t0 + float(i) * r.step;
-            // a linear TF's interval is located once for the value and the adjoint
-            TfLoc L{};
-            if constexpr (TFM == 2) L = tf_linear_locate<true>(T, dv[j]);
-            const f4 s = tf_eval<TFM, true>(T, dv[j], &L);
+            // a linear TF's interval and the Gaussian weights are formed once for the value and the adjoint
+            TfPre pre;
+            tf_prepare<TFM, true>(T, dv[j], pre);
+            const f4 s = tf_eval<TFM, true>(T, dv[j], &pre);
             f4 adj_s;
             float adj_st;
             if constexpr (kKeepAK) blend_adjoint_ak<BLEND>(tin[j], s, r.step, al[j], kp[j], adj_acc, adj_s, adj_st);
             else blend_adjoint_t<BLEND>(tin[j], s, r.step, adj_acc, adj_s, adj_st);
             float adj_d;
-            if (wantTf) adj_d = tf_adjoint<TFM, CT_TF, Sink, true>(T, dv[j], adj_s, sink, &L);
+            if (wantTf) adj_d = tf_adjoint<TFM, CT_TF, Sink, true>(T, dv[j], adj_s, sink, &pre);
             else {
                 SinkNone none;
-                adj_d = tf_adjoint<TFM, false, SinkNone, true>(T, dv[j], adj_s, none, &L);
+                adj_d = tf_adjoint<TFM, false, SinkNone, true>(T, dv[j], adj_s, none, &pre);
             }
             if constexpr (CT_VOL || CT_POS) {
                 if (wantVol || wantPos) {
