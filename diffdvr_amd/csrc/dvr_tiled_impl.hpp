@@ -514,10 +514,9 @@ __device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs
     const int ddx = ix - c.cx, ddy = iy - c.cy, ddz = iz - c.cz;
     const bool valid = c.cx != kInvalidCell;
     const bool unit = valid && (abs(ddx) + abs(ddy) + abs(ddz)) == 1;
+    // written by the case a lane takes before it reads them (the upper four only by the jumps)
     unsigned lo[8];
     float lv[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) { lo[k] = 0u; lv[k] = 0.f; }
     if (unit) {
         CornerIdx oldG{};
         if constexpr (VOL) oldG = vs.L.cell(c.cx, c.cy, c.cz);
@@ -530,11 +529,11 @@ __device__ __forceinline__ void corner_move(CornerCache& c, const Vol& V, VS& vs
         else corner_shift<1, false, VOL, POS>(c, V, oldG, newV, lo, lv);
     } else {
         if constexpr (VOL) {
-            if (valid) {
-                const CornerIdx o = vs.L.cell(c.cx, c.cy, c.cz);
+            // a ray's first cell (no current cell yet) holds zero gradients, which claim and add
+            // nothing; its corner offsets are formed from the clamped invalid index and never used
+            const CornerIdx o = vs.L.cell(valid ? c.cx : 0, valid ? c.cy : 0, valid ? c.cz : 0);
 #pragma unroll
-                for (int k = 0; k < 8; ++k) { lo[k] = corner_at(o, k); lv[k] = c.w[k]; }
-            }
+            for (int k = 0; k < 8; ++k) { lo[k] = corner_at(o, k); lv[k] = c.w[k]; }
 #pragma unroll
             for (int k = 0; k < 8; ++k) c.w[k] = 0.f;
         }
