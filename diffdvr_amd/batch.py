@@ -200,7 +200,10 @@ class BatchPlan:
         self._h.set_options(int(checkpoint_interval), int(wave_width), int(grid), not density_store)
 
     def autotune(self, iterations: int = 3) -> dict:
-        """Time every (K, wave width) candidate on this batch, keep the fastest.
+        """Time the candidates on this batch and keep the fastest: checkpoint interval K in
+        {4, 8, 16, 32, 64} jointly with the persistent workgroups (1-4 waves per SIMD and full
+        residency, per K), then the wave width (tile 4x16, 8x8, 16x4 or 32x2).  Results are cached
+        per device and batch signature (in the process and a file, see dvr_batch_plan_autotune).
 
         Outputs are overwritten and the gradient buffers left zeroed."""
         k, w, g, ri = self._h.autotune(int(iterations), self._stream())
