@@ -128,6 +128,9 @@ struct dvr_batch_plan {
         return size_t(dvr::kTfCopies) * st * sizeof(float) <= 16384 ? st : 0;
     }
 
+    // floats of the LDS TF table (the kernel pads linear-TF rows to 8 floats, dvr::tf_lds_stride)
+    int tf_lds() const { return tfm == 2 ? (maxRC / 5) * 8 : maxRC; }
+
     // register segment length of checkpoint interval K (64 = two 32-sample segments)
     int segK() const { return K > 32 ? 32 : K; }
 
@@ -143,7 +146,7 @@ struct dvr_batch_plan {
         P.wantTf = (gm & 1) ? 1 : 0; P.wantVol = (gm & 2) ? 1 : 0; P.wantCam = (gm & 4) ? 1 : 0;
         P.waveShift = wave_shift_of(waveWidth);
         P.tileW = waveWidth; P.tileH = 64 / waveWidth;
-        P.ldsTf = maxRC; P.usePriv = usePriv ? 1 : 0;
+        P.ldsTf = tf_lds(); P.usePriv = usePriv ? 1 : 0;
         P.tfCopyStride = (gm && !usePriv) ? copy_stride() : 0;
         P.ldsAgg = aggOffset;
         P.aggLog2 = aggLog2;
@@ -168,7 +171,7 @@ struct dvr_batch_plan {
             gm ? (usePriv ? size_t(maxRC) * dvr::kBatchBlock
                           : (copy_stride() > 0 ? size_t(dvr::kTfCopies) * copy_stride() : size_t(maxRC)))
                : 0;
-        lds = sizeof(float) * (size_t(maxRC) + ldsG);
+        lds = sizeof(float) * (size_t(tf_lds()) + ldsG);
         // volume gradients: the per-wave table of 64-byte chunks follows the TF tables (not
         // reserved for camera-only plans, which run the same instance without it)
         aggOffset = 0;
@@ -176,7 +179,7 @@ struct dvr_batch_plan {
         if ((gm & 2) && volGrad) {
             // the largest table that keeps the instance at the waves per SIMD its registers
             // were fitted for (workgroups per CU from the runtime's occupancy calculator)
-            aggOffset = dvr::agg_offset(maxRC, (int)ldsG);
+            aggOffset = dvr::agg_offset(tf_lds(), (int)ldsG);
             const int target = 4 * dvr::batched_vol_waves(gm);
             for (int lg = dvr::kAggLog2; lg >= dvr::kAggMinLog2; --lg) {
                 aggLog2 = lg;

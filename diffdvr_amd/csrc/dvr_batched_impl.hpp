@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(
     GM == 0 ? 5 : ((GM & 2) ? batched_vol_waves(GM) : 2)))) void k_batched(BatchParams B)
 {
     constexpr bool ADJ = GM != 0;
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ int s_tile;
     float* s_tf = lds;
     float* s_g = lds + B.ldsTf;
@@ -118,7 +118,9 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(
             curTf = tfi;
             curRC = tf.R * tf.C;
             curWantTf = ADJ && B.wantTf && tf.adj != nullptr;
-            for (int e = threadIdx.x; e < curRC; e += kBatchBlock) s_tf[e] = gld(tf.p, unsigned(e));
+            // rows at a stride of tf_lds_stride<TFM>() floats (16-byte aligned RGBA for the linear TF)
+            for (int e = threadIdx.x; e < curRC; e += kBatchBlock)
+                s_tf[(e / tf_cols<TFM>()) * tf_lds_stride<TFM>() + (e % tf_cols<TFM>())] = gld(tf.p, unsigned(e));
             if (curWantTf) {
                 const int ng = priv ? curRC * kBatchBlock : (B.tfCopyStride > 0 ? kTfCopies * B.tfCopyStride : curRC);
                 for (int e = threadIdx.x; e < ng; e += kBatchBlock) s_g[e] = 0.0f;
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(
         const int x = tx * B.tileW + lx, y = ty * B.tileH + ly;
         const bool valid = x < V.W && y < V.H;
         constexpr int kC = tf_cols<TFM>();  // validated: every TF of the plan has this many columns
-        const Tf T{s_tf, curRC / kC, kC, kC, 1};
+        const Tf T{s_tf, curRC / kC, kC, tf_lds_stride<TFM>(), 1};
         SinkTiled<TFM> sink;
         sink.init(s_g, kC, priv, kBatchBlock, priv ? 0 : B.tfCopyStride);
         const bool wantCam = ADJ && (GM & 4) && B.wantCam && V.acam != nullptr;

@@ -426,6 +426,13 @@ constexpr int tf_cols()
 {
     return TFM == 0 ? 2 : (TFM == 1 ? 4 : (TFM == 2 ? 5 : 6));
 }
+// Row stride of a TF staged in LDS by the batched kernel: the linear TF's rows
+// are padded to 8 floats so a row's RGBA is one 16-byte LDS read.
+template <int TFM>
+constexpr int tf_lds_stride()
+{
+    return TFM == 2 ? 8 : tf_cols<TFM>();
+}
 
 // Interval of a piecewise-linear TF that holds a density: rows i and i+1, their
 // control-point positions, and (FAST) the reciprocal of the interval width.

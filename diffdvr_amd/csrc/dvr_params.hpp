@@ -160,7 +160,7 @@ struct BatchParams {
     int wantTf, wantVol, wantCam;
     int waveShift;              // log2 of the wave tile width; wave tile = (1<<s) x (64>>s) rays
     int tileW, tileH;           // workgroup tile = one wave tile
-    int ldsTf;                  // floats of LDS per TF table (max R*C over the TFs)
+    int ldsTf;                  // floats of the LDS TF table (max rows x the staged row stride)
     int usePriv;                // Gaussian TF gradient in per-thread LDS slices
     int tfCopyStride;           // floats between the kTfCopies TF-gradient tables (0: one table)
     int ldsAgg;                 // float offset of the volume-gradient chunk table in LDS (0: none)
