@@ -444,18 +444,24 @@ struct TfLoc {
     int i;
     float p0, p1, inv;
 };
+// The interval that starts at row i (a density's interval once its row is known).
 template <bool FAST>
-__device__ __forceinline__ TfLoc tf_linear_locate(const Tf& T, float density)
+__device__ __forceinline__ TfLoc tf_linear_row(const Tf& T, int i)
 {
-    int i;
-    for (i = 0; i < T.R - 2; ++i)
-        if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
     TfLoc L;
     L.i = i;
     L.p0 = T.p[i * T.sr + 4 * T.sc];
     L.p1 = T.p[(i + 1) * T.sr + 4 * T.sc];
     L.inv = FAST ? __builtin_amdgcn_rcpf(L.p1 - L.p0) : 0.0f;
     return L;
+}
+template <bool FAST>
+__device__ __forceinline__ TfLoc tf_linear_locate(const Tf& T, float density)
+{
+    int i;
+    for (i = 0; i < T.R - 2; ++i)
+        if (T.p[(i + 1) * T.sr + 4 * T.sc] > density) break;
+    return tf_linear_row<FAST>(T, i);
 }
 
 // One Gaussian component's weight exp(-(d - mu)^2 / (2 s^2)); FAST multiplies by
@@ -533,8 +539,9 @@ __device__ __forceinline__ f4 tf_eval(const Tf& T, float density, const TfPre* p
 }
 
 // Absorption channel of the TF only (the transmittance recompute needs no colour).
+// A linear TF's interval is located here unless the caller passes it in `loc`.
 template <int TFM, bool FAST = false>
-__device__ __forceinline__ float tf_absorption(const Tf& T, float density)
+__device__ __forceinline__ float tf_absorption(const Tf& T, float density, const TfLoc* loc = nullptr)
 {
     if constexpr (TFM == 0) {
         return fclamp(density, 0.0f, 1.0f) * T.p[0];
@@ -546,7 +553,7 @@ __device__ __forceinline__ float tf_absorption(const Tf& T, float density)
         const float c = T.p[iclamp(di + 1, 0, T.R - 1) * T.sr + 3 * T.sc];
         return lerpf_(a, c, df);
     } else if constexpr (TFM == 2) {
-        const TfLoc L = tf_linear_locate<FAST>(T, density);
+        const TfLoc L = loc ? *loc : tf_linear_locate<FAST>(T, density);
         const float* a = T.p + L.i * T.sr;
         const float* c = a + T.sr;
         const float d2 = fclamp(density, L.p0, L.p1);

@@ -579,8 +579,14 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
     // per SIMD and need the registers)
     constexpr bool kVolOnly = CT_VOL && !CT_POS;
     constexpr bool kKeepAK = BLEND == 0 && K <= 16 && !kVolOnly;
+    // a linear TF's interval row per sample, located once by the recompute and reused by the walk
+    // (in the volume-gradient instances, whose walk is a rolled loop): two 16-bit rows per
+    // register (a linear TF has at most kMaxTfLds / 5 < 65536 rows)
+    constexpr bool kKeepRow = TFM == 2 && kVolOnly;
+    static_assert(kMaxTfLds / 5 < 65536, "TF rows must fit 16 bits");
     float dv[K], tin[K];
     float al[kKeepAK ? K : 1], kp[kKeepAK ? K : 1];
+    unsigned rows[kKeepRow ? (K + 1) / 2 : 1];
     // reverse-walk cell state for the volume / position adjoint
     CornerCache cc;
     cc.cx = kInvalidCell; cc.cy = 0; cc.cz = 0;
@@ -623,7 +629,14 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
             for (int j = 0; j < K; ++j) {
                 tin[j] = Tr;
                 float alpha, keep;
-                alpha_keep<BLEND>(tf_absorption<TFM, true>(T, dv[j]), r.step, alpha, keep);
+                if constexpr (kKeepRow) {
+                    const TfLoc L = tf_linear_locate<true>(T, dv[j]);
+                    if (j & 1) rows[j >> 1] |= unsigned(L.i) << 16;
+                    else rows[j >> 1] = unsigned(L.i);
+                    alpha_keep<BLEND>(tf_absorption<TFM, true>(T, dv[j], &L), r.step, alpha, keep);
+                } else {
+                    alpha_keep<BLEND>(tf_absorption<TFM, true>(T, dv[j]), r.step, alpha, keep);
+                }
                 if constexpr (kKeepAK) { al[j] = alpha; kp[j] = keep; }
                 Tr *= keep;
             }
@@ -640,7 +653,14 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
                     if (more) pipe.request(V, pn);
                     tin[j] = Tr;
                     float alpha, keep;
-                    alpha_keep<BLEND>(tf_absorption<TFM, true>(T, dv[j]), r.step, alpha, keep);
+                    if constexpr (kKeepRow) {
+                        const TfLoc L = tf_linear_locate<true>(T, dv[j]);
+                        if (j & 1) rows[j >> 1] |= unsigned(L.i) << 16;
+                        else rows[j >> 1] = unsigned(L.i);
+                        alpha_keep<BLEND>(tf_absorption<TFM, true>(T, dv[j], &L), r.step, alpha, keep);
+                    } else {
+                        alpha_keep<BLEND>(tf_absorption<TFM, true>(T, dv[j]), r.step, alpha, keep);
+                    }
                     if constexpr (kKeepAK) { al[j] = alpha; kp[j] = keep; }
                     Tr *= keep;
                     if (more) pipe.advance();
@@ -656,7 +676,8 @@ __device__ __forceinline__ void march_reverse(const Vol& V, const Tf& T, const R
             const float t = r.t0 + float(i) * r.step;
             // a linear TF's interval and the Gaussian weights are formed once for the value and the adjoint
             TfPre pre;
-            tf_prepare<TFM, true>(T, dv[j], pre);
+            if constexpr (kKeepRow) pre.L = tf_linear_row<true>(T, int((rows[j >> 1] >> ((j & 1) * 16)) & 0xffffu));
+            else tf_prepare<TFM, true>(T, dv[j], pre);
             const f4 s = tf_eval<TFM, true>(T, dv[j], &pre);
             f4 adj_s;
             float adj_st;
