@@ -360,25 +360,26 @@ __device__ __forceinline__ float fetch(const Vol& V, f3 p, Cell& c)
 // rays enters a new cell on almost every step, so a plain cell cache issues
 // and waits for loads each sample).  Positions handed to request() must be
 // the ones later passed to value(): callers carry them from one iteration
-// to the next.
+// to the next, and call value(p) before request(V, pn): the next cell's
+// loads land in the registers of the current cell, whose values are no longer
+// needed once p is interpolated (one set of 8 corner registers, not two).
 //   start(V, p)   load the cell of the first position
 //   value(p)      interpolate at p (p lies in the current cell)
-//   request(V, pn) issue the loads of the cell of pn if it differs
-//   advance()     make pn's cell current (waits for the loads)
+//   request(V, pn) issue the loads of the cell of pn if it differs; pn's cell
+//                 is current from here on (its first use waits for the loads)
+//   advance()     nothing left to do (kept for the nearest filter's pipe)
 // ---------------------------------------------------------------------------
 template <int FILTER>
 struct FetchPipe;
 
 template <>
 struct FetchPipe<1> {
-    int ix, iy, iz, jx, jy, jz;
-    bool chg;
-    float d[8], nd[8];
+    int ix, iy, iz;
+    float d[8];
     __device__ __forceinline__ void start(const Vol& V, f3 p)
     {
         ix = int(p.x); iy = int(p.y); iz = int(p.z);
         cell_load(V, ix, iy, iz, d);
-        chg = false;
     }
     __device__ __forceinline__ float value(f3 p) const
     {
@@ -386,18 +387,13 @@ struct FetchPipe<1> {
     }
     __device__ __forceinline__ void request(const Vol& V, f3 pn)
     {
-        jx = int(pn.x); jy = int(pn.y); jz = int(pn.z);
-        chg = (jx != ix) | (jy != iy) | (jz != iz);
-        if (chg) cell_load(V, jx, jy, jz, nd);
-    }
-    __device__ __forceinline__ void advance()
-    {
-        if (chg) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) d[k] = nd[k];
+        const int jx = int(pn.x), jy = int(pn.y), jz = int(pn.z);
+        if ((jx != ix) | (jy != iy) | (jz != iz)) {
+            cell_load(V, jx, jy, jz, d);
             ix = jx; iy = jy; iz = jz;
         }
     }
+    __device__ __forceinline__ void advance() {}
 };
 
 template <>
